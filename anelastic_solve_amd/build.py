@@ -1,0 +1,78 @@
+"""Build the gfx950 shared library ``libanelastic_mg.so`` in-tree with hipcc.
+
+Usage: ``python -m anelastic_solve_amd.build`` (or ``build()`` from Python).
+The library is the product: the HIP kernels, the host orchestrator and the
+C ABI of ``include/amg.h``.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libanelastic_mg.so")
+
+SOURCES = ["amg_kernels.hip", "amg_hierarchy.cpp", "amg_solver.cpp", "amg_server.cpp",
+           "amg_capi.cpp"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
+    "-Wno-unused-function", "-Wno-unused-result",
+    # packed-FP32 instructions stay off (see the design notes)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose=False, jobs=4):
+    """Compile every source for gfx950 and link the shared library."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [HIPCC, "-x", "hip", *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                            text=True)))
+        objs.append(obj)
+        if len(procs) >= jobs:
+            _wait(procs.pop(0))
+    for p in procs:
+        _wait(p)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def _wait(item):
+    cmd, p = item
+    out, err = p.communicate()
+    if p.returncode != 0:
+        sys.stderr.write(out + err)
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    msg = (out + err).strip()
+    if msg:
+        # keep real warnings visible, drop the host pass's note on the packed-fp32 feature
+        lines = [l for l in msg.splitlines() if "packed-fp32-ops" not in l]
+        if lines:
+            sys.stderr.write("\n".join(lines) + "\n")
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

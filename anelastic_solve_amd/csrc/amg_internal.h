@@ -1,0 +1,143 @@
+// amg_internal.h -- shared device descriptors and kernel launchers.
+//
+// Field layout in HBM ("chequer" layout).  A level with nx*ny*nz interior
+// cells (nx even) stores each x-row (j,k) as two contiguous half-rows: the
+// cells of column colour c = (i+j)&1, ordered by m = i>>1:
+//
+//     index(i,j,k) = ((k*ny + j)*2 + ((i+j)&1))*hx + (i>>1),   hx = nx/2.
+//
+// Both red-black smoothers then stream contiguous memory: a point-RB colour
+// in plane k is one column colour ((i+j+k)&1 == p  <=>  (i+j)&1 == (p+k)&1),
+// a z-line colour is one column colour in every plane, and all six
+// neighbours of a cell are at the same m (y, z) or at m-1+r, m+r (x, with
+// r = i&1) of a half-row of known colour.  No ghost cells are stored:
+// periodic directions wrap the index, walls carry zero coupling in the 1-D
+// coefficient tables.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace amg {
+
+// Coefficient table of one level (precision T): 3*nx + 3*ny + 3*nz values,
+//   [wxm | wxp | dx] [wym | wyp | dy] [wzm | wzp | dz]
+// with (A u)_ijk = wxm_i u_{i-1} + wxp_i u_{i+1} + ... + (dx_i+dy_j+dz_k) u_ijk.
+struct DevLevel {
+  int nx, ny, nz, hx;
+  int px, py, pz;            // periodic flags
+  int inst;                  // instance id (batched/served runs)
+  int ncell;                 // nx*ny*nz
+  int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
+  int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
+  double vsum;               // finest: sum of cell volumes
+  double vrsum;              // finest: sum of V*rho
+  const double* co64;        // coefficient table, FP64
+  const float* co32;         // coefficient table, FP32
+  const double* vol;         // [dxc(nx) | dyc(ny) | dzc(nz)] interior widths
+  const double* rho;         // rho(z) interior, nz values
+  const double* bcfold;      // [x(nx) | y(ny) | z(nz)] boundary-value RHS shifts (finest only)
+  const double* minvT;       // coarsest level: transposed dense (pseudo-)inverse, ncell^2
+  void* u;                   // solution / correction (chequer layout)
+  void* u2;                  // FP64 finest: ping-pong partner of u
+  void* f;                   // right-hand side (chequer layout)
+  void* scr;                 // z-line scratch: nz * hx * ny values
+};
+
+// One kernel launch over several items (levels of different instances),
+// passed by value in the kernel arguments (no device upload, so a served
+// batch can change every round).  Block b belongs to the item t with
+// prefix[t] <= b < prefix[t+1].
+constexpr int kMaxItems = 128;
+struct LaunchList {
+  int nitems;
+  int nblocks;
+  int items[kMaxItems];        // item -> DevLevel index (or transfer index)
+  int prefix[kMaxItems + 1];   // block prefix
+};
+
+// Per-item caller arrays (reference-layout fields), by value.
+struct PtrList {
+  double* p[kMaxItems];
+};
+
+// Fine->coarse pair for transfers.
+struct DevXfer {
+  int fine, coarse;
+  int fx, fy, fz;
+};
+
+constexpr int kBlock = 256;
+
+void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
+
+// ---- V-cycle kernels (amg_kernels.hip); fp64 selects the level precision --
+void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                     int zero_guess, const int* active, hipStream_t s);
+void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                  int zero_guess, const int* active, hipStream_t s);
+void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                     const int* active, hipStream_t s);
+void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                    const int* active, hipStream_t s);
+void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, const int* active,
+                   hipStream_t s);
+
+// ---- FP64 finest-level kernels (items index FP64 finest entries) -------------
+// Each block writes nvals partial maxima to partial[block*nvals + v];
+// launch_finalize reduces them per item into out[item*ostride + v].
+// residual_max:     {max|b - A u|, max|b|}
+// change_max:       {max|u - u2|, max|u|}        (u2 holds the previous iterate)
+// defect32:         r = b - A u -> FP32 f of entry 'link'; {max|r|, max|b|}
+// correct_defect32: u2 = u + e (e = FP32 u of 'link'); r = b - A u2 -> FP32 f;
+//                   {max|r|, max|e|, max|u2|}
+void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial,
+                         const int* active, hipStream_t s);
+void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial,
+                       const int* active, hipStream_t s);
+void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
+                     const int* active, hipStream_t s);
+void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
+                             const int* active, hipStream_t s);
+void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum,
+                     double* out, int ostride, hipStream_t s);
+
+// Conversions between the reference's ghosted Fortran arrays (one pointer
+// per item in 'ptr') and the FP64 finest level.
+// from_fortran: dst = (which==0 ? u : f) = src interior (+ bcfold if add_fold)
+// wsum:         {sum V x} (x = u or f), or {sum V rho} when which == 2
+// shift:        x -= (s[item] / (with_rho ? vrsum : vsum)) * (with_rho ? rho(z) : 1)
+//               with s the weighted sums of launch_wsum (after finalize)
+// to_fortran:   dst interior = u
+// copy_u:       u2 = u
+void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr,
+                         int which, int add_fold, hipStream_t s);
+void launch_wsum(const DevLevel* L, const LaunchList& ll, int which, double* partial,
+                 hipStream_t s);
+void launch_shift(const DevLevel* L, const LaunchList& ll, int which, const double* c,
+                  int with_rho, hipStream_t s);
+void launch_to_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr,
+                       hipStream_t s);
+void launch_copy_u(const DevLevel* L, const LaunchList& ll, const int* active, hipStream_t s);
+
+// Ghost layers of a ghosted FP64 array (the serial path of mgMK/mg_bdry.F90).
+void launch_ghost_wrap(double* a, int nx, int ny, int nz, int dir, hipStream_t s);
+void launch_ghost_face(double* a, int nx, int ny, int nz, int face, int type, double v,
+                       double dedge, hipStream_t s);
+
+// Projection step (reference layout, FP64).
+void launch_psource(const double* U, const double* V, const double* W, double* SRC,
+                    const double* rho, const double* rdxc, const double* rdyc,
+                    const double* rdzc, int nx, int ny, int nz, double rdt, hipStream_t s);
+// wallmask bit d (x=1, y=2, z=4): the upper face of direction d is a
+// pressure-Neumann wall whose normal velocity is left to the velocity BC.
+void launch_pgrad(const double* p, double* U, double* V, double* W, const double* rho,
+                  const double* rdxe, const double* rdye, const double* rdze, int nx, int ny,
+                  int nz, double dt, int wallmask, hipStream_t s);
+int divergence_blocks(int nx, int ny, int nz);
+void launch_divergence(const double* U, const double* V, const double* W, double* div,
+                       const double* rdxc, const double* rdyc, const double* rdzc, int nx,
+                       int ny, int nz, double* partial, hipStream_t s);
+void launch_divergence_finalize(const double* partial, int nblocks, double* out, hipStream_t s);
+
+}  // namespace amg

@@ -1,0 +1,819 @@
+// amg_kernels.hip -- gfx950 kernels of the multigrid pressure solve.
+//
+// Every V-cycle kernel runs over a work list: one launch covers the same
+// step of several instances/levels (LaunchList in amg_internal.h), so a
+// single solve and a batch of uneven solves share the same code.  All
+// kernels are HBM-streaming stencils (about 0.5 flop/byte): no MFMA.
+#include "amg_internal.h"
+
+#include <cmath>
+
+namespace amg {
+namespace {
+
+__device__ __forceinline__ int find_item(const int* prefix, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class T> __device__ __forceinline__ const T* coefs(const DevLevel& L);
+template <> __device__ __forceinline__ const double* coefs<double>(const DevLevel& L) { return L.co64; }
+template <> __device__ __forceinline__ const float* coefs<float>(const DevLevel& L) { return L.co32; }
+
+// NaN-propagating maximum (fmax ignores NaN).
+__device__ __forceinline__ double nmax(double a, double b) {
+  return (a != a) ? a : ((b != b) ? b : fmax(a, b));
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide reduction of NV values; thread 0..NV-1 writes out[q].
+template <int NV, bool SUM>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __restrict__ out) {
+  __shared__ double sm[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = SUM ? wave_sum(v[q]) : wave_max(v[q]);
+    if (lane == 0) sm[q][w] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double a = sm[threadIdx.x][0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+      a = SUM ? a + sm[threadIdx.x][i] : nmax(a, sm[threadIdx.x][i]);
+    out[threadIdx.x] = a;
+  }
+}
+
+// Chequer-layout index of (m, j, k) in the half-row of colour c.
+__device__ __forceinline__ size_t cidx(int m, int j, int k, int c, int hx, int ny) {
+  return ((size_t)(k * ny + j) * 2 + c) * hx + m;
+}
+
+// Seven chequer indices of cell (i,j,k): centre, x-, x+, y-, y+, z-, z+.
+struct Nb {
+  size_t c, xm, xp, ym, yp, zm, zp;
+};
+
+__device__ __forceinline__ Nb neighbours(const DevLevel& L, int i, int j, int k) {
+  const int hx = L.hx, ny = L.ny, nz = L.nz;
+  const int col = (i + j) & 1, oth = col ^ 1, m = i >> 1, r = i & 1;
+  int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
+  int mr = m + r;     if (mr >= hx) mr = 0;
+  const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+  const int km = (k == 0) ? nz - 1 : k - 1, kp = (k == nz - 1) ? 0 : k + 1;
+  Nb n;
+  n.c = cidx(m, j, k, col, hx, ny);
+  const size_t ro = cidx(0, j, k, oth, hx, ny);
+  n.xm = ro + ml;
+  n.xp = ro + mr;
+  n.ym = cidx(m, jm, k, oth, hx, ny);
+  n.yp = cidx(m, jp, k, oth, hx, ny);
+  n.zm = cidx(m, j, km, col, hx, ny);
+  n.zp = cidx(m, j, kp, col, hx, ny);
+  return n;
+}
+
+// Decode a chequer element index into (i, j, k).
+__device__ __forceinline__ void decode(const DevLevel& L, int e, int& i, int& j, int& k) {
+  const int m = e % L.hx;
+  const int t = e / L.hx;
+  const int c = t & 1;
+  const int row = t >> 1;
+  j = row % L.ny;
+  k = row / L.ny;
+  i = 2 * m + ((c + j) & 1);
+}
+
+// Off-diagonal stencil sum, in a fixed order: x-, x+, y-, y+, z-, z+.
+template <class T>
+__device__ __forceinline__ T offsum(const T* __restrict__ co, const DevLevel& L, int i, int j, int k,
+                                    T uxm, T uxp, T uym, T uyp, T uzm, T uzp) {
+  const int nx = L.nx, ny = L.ny;
+  const T* cy = co + 3 * nx;
+  const T* cz = cy + 3 * ny;
+  T s = co[i] * uxm + co[nx + i] * uxp;
+  s += cy[j] * uym;
+  s += cy[ny + j] * uyp;
+  s += cz[k] * uzm;
+  s += cz[L.nz + k] * uzp;
+  return s;
+}
+
+template <class T>
+__device__ __forceinline__ T diag_at(const T* __restrict__ co, const DevLevel& L, int i, int j, int k) {
+  const int nx = L.nx, ny = L.ny;
+  const T* cy = co + 3 * nx;
+  const T* cz = cy + 3 * ny;
+  return (cz[2 * L.nz + k] + cy[2 * ny + j]) + co[2 * nx + i];
+}
+
+// ---------------------------------------------------------------------------
+// Point red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:146-168):
+// u = (f - sum w_n u_n)/diag on the cells with (i+j+k)&1 == parity.  Thread
+// per cell of the colour; in plane k the colour is column colour (parity+k)&1,
+// so each thread reads one contiguous half-row position of f and u.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                     int parity, int zero_guess,
+                                                     const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int hx = L.hx, ny = L.ny;
+  if (e >= hx * ny * L.nz) return;
+  const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
+  const int c = (k + parity) & 1;
+  const int i = 2 * m + ((c + j) & 1);
+  const T* __restrict__ co = coefs<T>(L);
+  T* __restrict__ u = static_cast<T*>(L.u);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const Nb n = neighbours(L, i, j, k);
+  const T d = diag_at<T>(co, L, i, j, k);
+  T val;
+  if (zero_guess) {
+    val = f[n.c] / d;
+  } else {
+    const T s = offsum<T>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
+    val = (f[n.c] - s) / d;
+  }
+  u[n.c] = val;
+}
+
+// ---------------------------------------------------------------------------
+// z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-538 with
+// Thomas :541-595): thread per z-column of colour (i+j)&1 == parity; the
+// tridiagonal system wzm u_{k-1} + (dx+dy+dz) u_k + wzp u_{k+1} = f - (x/y
+// neighbour terms) is solved exactly.  Forward elimination keeps the
+// modified RHS in u (overwritten by back substitution) and the modified
+// super-diagonal in the scratch array.  A periodic z wrap is lagged.
+template <class T>
+__global__ __launch_bounds__(64) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                              int parity, int zero_guess,
+                                              const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int col = (blockIdx.x - ll.prefix[t]) * 64 + threadIdx.x;
+  const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
+  const int ncol = hx * ny;
+  if (col >= ncol) return;
+  const int m = col % hx, j = col / hx;
+  const int c = parity, oth = parity ^ 1;
+  const int i = 2 * m + ((c + j) & 1), r = i & 1;
+  int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
+  int mr = m + r;     if (mr >= hx) mr = 0;
+  const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+  const T* __restrict__ co = coefs<T>(L);
+  const T* cy = co + 3 * nx;
+  const T* cz = cy + 3 * ny;
+  const T wxl = co[i], wxr = co[nx + i], wys = cy[j], wyn = cy[ny + j];
+  const T dxy = co[2 * nx + i] + cy[2 * ny + j];
+  T* __restrict__ u = static_cast<T*>(L.u);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  T* __restrict__ cp = static_cast<T*>(L.scr);
+  const size_t plane = (size_t)ny * 2 * hx;
+  const size_t own = cidx(m, j, 0, c, hx, ny);
+  const size_t xo = cidx(0, j, 0, oth, hx, ny);
+  const size_t so = cidx(m, jm, 0, oth, hx, ny);
+  const size_t no = cidx(m, jp, 0, oth, hx, ny);
+  const bool pz = L.pz != 0;
+  T lagb = T(0), lagt = T(0);
+  if (pz && !zero_guess) {
+    lagb = cz[0] * u[(size_t)(nz - 1) * plane + own];
+    lagt = cz[nz + nz - 1] * u[own];
+  }
+  T cprev = T(0), dprev = T(0);
+  for (int k = 0; k < nz; ++k) {
+    const size_t kb = (size_t)k * plane;
+    T g = f[kb + own];
+    if (!zero_guess) {
+      T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
+      s += wys * u[kb + so];
+      s += wyn * u[kb + no];
+      g = g - s;
+    }
+    T a = cz[k], cc = cz[nz + k];
+    if (pz) {
+      if (k == 0) { g -= lagb; a = T(0); }
+      if (k == nz - 1) { g -= lagt; cc = T(0); }
+    }
+    const T den = (cz[2 * nz + k] + dxy) - a * cprev;
+    const T cpk = cc / den;
+    const T dpk = (g - a * dprev) / den;
+    cp[(size_t)k * ncol + col] = cpk;
+    u[kb + own] = dpk;
+    cprev = cpk;
+    dprev = dpk;
+  }
+  T x = dprev;
+  for (int k = nz - 2; k >= 0; --k) {
+    const size_t kb = (size_t)k * plane;
+    x = u[kb + own] - cp[(size_t)k * ncol + col] * x;
+    u[kb + own] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused residual + volume-weighted restriction (mgMK/mg_restr.F90:124-164):
+// thread per coarse cell in natural (I fastest) order, so the fine reads of
+// a wave are contiguous half-rows; the fine residual is never stored.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict__ Lt,
+                                                     const DevXfer* __restrict__ X, LaunchList ll,
+                                                     const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& F = Lt[x.fine];
+  const DevLevel& C = Lt[x.coarse];
+  if (active && !active[F.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= C.ncell) return;
+  const int I = e % C.nx, JK = e / C.nx, J = JK % C.ny, K = JK / C.ny;
+  const T* __restrict__ co = coefs<T>(F);
+  const T* __restrict__ u = static_cast<const T*>(F.u);
+  const T* __restrict__ f = static_cast<const T*>(F.f);
+  const double* vx = F.vol;
+  const double* vy = vx + F.nx;
+  const double* vz = vy + F.ny;
+  T acc = T(0), vs = T(0);
+  for (int cz = 0; cz < x.fz; ++cz) {
+    const int k = K * x.fz + cz;
+    for (int cy = 0; cy < x.fy; ++cy) {
+      const int j = J * x.fy + cy;
+      for (int cx = 0; cx < x.fx; ++cx) {
+        const int i = I * x.fx + cx;
+        const Nb n = neighbours(F, i, j, k);
+        const T s = offsum<T>(co, F, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
+        const T rr = f[n.c] - (s + diag_at<T>(co, F, i, j, k) * u[n.c]);
+        const T v = T(vz[k] * vy[j] * vx[i]);
+        acc += v * rr;
+        vs += v;
+      }
+    }
+  }
+  T* __restrict__ fc = static_cast<T*>(C.f);
+  fc[cidx(I >> 1, J, K, (I + J) & 1, C.hx, C.ny)] = acc / vs;
+}
+
+// 1-D prolongation weights of fine index i: coarse (I0, w0), (I1, w1).
+__device__ __forceinline__ void pw(int i, int nc, int fac, int per, int bc0, int bc1, int& I0,
+                                   int& I1, double& w0, double& w1) {
+  if (fac == 1) { I0 = I1 = i; w0 = 1.0; w1 = 0.0; return; }
+  I0 = i >> 1;
+  int far = (i & 1) ? I0 + 1 : I0 - 1;
+  w0 = 0.75; w1 = 0.25;
+  if (far < 0 || far >= nc) {
+    const int wall = (far < 0) ? bc0 : bc1;
+    if (per) far = (far + nc) % nc;
+    else if (wall == 2) { far = I0; w1 = 0.0; }
+    else far = I0;
+  }
+  I1 = far;
+}
+
+// ---------------------------------------------------------------------------
+// Trilinear prolongation-add (mgMK/mg_cor.F90:102-159): fine u += sum of
+// 27,9,9,3,9,3,3,1 /64 of the eight nearest coarse cells; coarse ghosts are
+// periodic wraps, Neumann copies or Dirichlet zeros.  Thread per fine cell
+// in chequer order (contiguous read-modify-write of the fine field).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__ Lt,
+                                                    const DevXfer* __restrict__ X, LaunchList ll,
+                                                    const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& F = Lt[x.fine];
+  const DevLevel& C = Lt[x.coarse];
+  if (active && !active[F.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= F.ncell) return;
+  int i, j, k;
+  decode(F, e, i, j, k);
+  int I[2], J[2], K[2];
+  double wx[2], wy[2], wz[2];
+  pw(i, C.nx, x.fx, F.px, F.bcx0, F.bcx1, I[0], I[1], wx[0], wx[1]);
+  pw(j, C.ny, x.fy, F.py, F.bcy0, F.bcy1, J[0], J[1], wy[0], wy[1]);
+  pw(k, C.nz, x.fz, F.pz, F.bcz0, F.bcz1, K[0], K[1], wz[0], wz[1]);
+  const T* __restrict__ uc = static_cast<const T*>(C.u);
+  T acc = T(0);
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    T sy = T(0);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const T v0 = uc[cidx(I[0] >> 1, J[b], K[a], (I[0] + J[b]) & 1, C.hx, C.ny)];
+      const T v1 = uc[cidx(I[1] >> 1, J[b], K[a], (I[1] + J[b]) & 1, C.hx, C.ny)];
+      const T sx = T(wx[0]) * v0 + T(wx[1]) * v1;
+      sy += T(wy[b]) * sx;
+    }
+    acc += T(wz[a]) * sy;
+  }
+  T* __restrict__ u = static_cast<T*>(F.u);
+  u[e] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level: u = Minv f with a precomputed FP64 (pseudo-)inverse
+// (replaces the ipre+ipost sweeps of mgMK/mg_cycle.F90:117-148).  Minv is
+// stored transposed so thread t's reads Minv^T[s][t] are coalesced.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                   const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int r = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int n = L.ncell;
+  if (r >= n) return;
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const double* __restrict__ M = L.minvT;
+  double acc = 0.0;
+  for (int s = 0; s < n; ++s) acc += M[(size_t)s * n + r] * (double)f[s];
+  static_cast<T*>(L.u)[r] = (T)acc;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 finest-level kernels.
+__global__ __launch_bounds__(kBlock) void k_residual_max(const DevLevel* __restrict__ Lt,
+                                                         LaunchList ll, double* __restrict__ partial,
+                                                         const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (e < L.ncell) {
+    int i, j, k;
+    decode(L, e, i, j, k);
+    const double* __restrict__ co = L.co64;
+    const double* __restrict__ u = static_cast<const double*>(L.u);
+    const double* __restrict__ f = static_cast<const double*>(L.f);
+    const Nb n = neighbours(L, i, j, k);
+    const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
+    v[0] = fabs(f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]));
+    v[1] = fabs(f[n.c]);
+  }
+  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restrict__ Lt,
+                                                       LaunchList ll, double* __restrict__ partial,
+                                                       const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (e < L.ncell) {
+    const double a = static_cast<const double*>(L.u)[e];
+    const double b = static_cast<const double*>(L.u2)[e];
+    v[0] = fabs(a - b);
+    v[1] = fabs(a);
+  }
+  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_defect32(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (e < L.ncell) {
+    int i, j, k;
+    decode(L, e, i, j, k);
+    const double* __restrict__ co = L.co64;
+    const double* __restrict__ u = static_cast<const double*>(L.u);
+    const double* __restrict__ f = static_cast<const double*>(L.f);
+    const Nb n = neighbours(L, i, j, k);
+    const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
+    const double r = f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]);
+    static_cast<float*>(Lt[L.link].f)[e] = (float)r;
+    v[0] = fabs(r);
+    v[1] = fabs(f[n.c]);
+  }
+  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+}
+
+// p_out = p_in + e, then r = b - A p_out from p_in and e at the seven points
+// (p_in is never written, so neighbouring threads read consistent values).
+__global__ __launch_bounds__(kBlock) void k_correct_defect32(const DevLevel* __restrict__ Lt,
+                                                             LaunchList ll,
+                                                             double* __restrict__ partial,
+                                                             const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (e < L.ncell) {
+    int i, j, k;
+    decode(L, e, i, j, k);
+    const DevLevel& S = Lt[L.link];
+    const double* __restrict__ co = L.co64;
+    const double* __restrict__ p = static_cast<const double*>(L.u);
+    const float* __restrict__ ev = static_cast<const float*>(S.u);
+    const double* __restrict__ b = static_cast<const double*>(L.f);
+    const Nb n = neighbours(L, i, j, k);
+#define PN(q) (p[n.q] + (double)ev[n.q])
+    const double pc = PN(c);
+    const double s = offsum<double>(co, L, i, j, k, PN(xm), PN(xp), PN(ym), PN(yp), PN(zm), PN(zp));
+#undef PN
+    const double r = b[n.c] - (s + diag_at<double>(co, L, i, j, k) * pc);
+    static_cast<double*>(L.u2)[n.c] = pc;
+    static_cast<float*>(S.f)[n.c] = (float)r;
+    v[0] = fabs(r);
+    v[1] = fabs((double)ev[n.c]);
+    v[2] = fabs(pc);
+  }
+  block_reduce_store<3, false>(v, partial + (size_t)blockIdx.x * 3);
+}
+
+// Per-item reduction of block partials (fixed order: deterministic).
+__global__ void k_finalize(LaunchList ll, const double* __restrict__ partial, int nvals, int op_sum,
+                           double* __restrict__ out, int ostride) {
+  const int t = blockIdx.x;
+  const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
+  for (int q = threadIdx.x; q < nvals; q += blockDim.x) {
+    double a = op_sum ? 0.0 : 0.0;
+    for (int b = b0; b < b1; ++b) {
+      const double x = partial[(size_t)b * nvals + q];
+      a = op_sum ? a + x : nmax(a, x);
+    }
+    out[(size_t)t * ostride + q] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions (FP64).
+__device__ __forceinline__ size_t fidx(int i, int j, int k, int nx, int ny) {
+  return ((size_t)(k + 1) * (ny + 2) + (j + 1)) * (nx + 2) + (i + 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restrict__ Lt,
+                                                         LaunchList ll, PtrList ptr,
+                                                         int which, int add_fold) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= L.ncell) return;
+  int i, j, k;
+  decode(L, e, i, j, k);
+  double v = ptr.p[t][fidx(i, j, k, L.nx, L.ny)];
+  if (add_fold) {
+    const double* bf = L.bcfold;
+    v += (bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k];
+  }
+  static_cast<double*>(which == 0 ? L.u : L.f)[e] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                 int which, double* __restrict__ partial) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  double v[1] = {0.0};
+  if (e < L.ncell) {
+    int i, j, k;
+    decode(L, e, i, j, k);
+    const double w = L.vol[L.nx + L.ny + k] * L.vol[L.nx + j] * L.vol[i];
+    const double x = (which == 2) ? L.rho[k]
+                                  : static_cast<const double*>(which == 0 ? L.u : L.f)[e];
+    v[0] = w * x;
+  }
+  block_reduce_store<1, true>(v, partial + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                  int which, const double* __restrict__ c,
+                                                  int with_rho) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= L.ncell) return;
+  double* x = static_cast<double*>(which == 0 ? L.u : L.f);
+  double s = c[t] / (with_rho ? L.vrsum : L.vsum);
+  if (with_rho) {
+    int i, j, k;
+    decode(L, e, i, j, k);
+    s *= L.rho[k];
+  }
+  x[e] -= s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restrict__ Lt,
+                                                       LaunchList ll, PtrList ptr) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= L.ncell) return;
+  int i, j, k;
+  decode(L, e, i, j, k);
+  ptr.p[t][fidx(i, j, k, L.nx, L.ny)] = static_cast<const double*>(L.u)[e];
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                   const int* __restrict__ active) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (active && !active[L.inst]) return;
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e >= L.ncell) return;
+  static_cast<double*>(L.u2)[e] = static_cast<const double*>(L.u)[e];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_fill_zero(const DevLevel* __restrict__ Lt, LaunchList ll) {
+  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e < L.ncell) static_cast<T*>(L.u)[e] = T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Ghost layers (serial path of mgMK/mg_bdry.F90:71-190).
+__global__ void k_ghost_wrap(double* __restrict__ a, int nx, int ny, int nz, int dir) {
+  const int sx = nx + 2, sy = ny + 2, sz = nz + 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dir == 0) {
+    if (e >= sy * sz) return;
+    const size_t base = (size_t)e * sx;
+    a[base + 0] = a[base + nx];
+    a[base + nx + 1] = a[base + 1];
+  } else if (dir == 1) {
+    if (e >= sx * sz) return;
+    const int i = e % sx, k = e / sx;
+    const size_t pl = (size_t)k * sy * sx;
+    a[pl + i] = a[pl + (size_t)ny * sx + i];
+    a[pl + (size_t)(ny + 1) * sx + i] = a[pl + (size_t)sx + i];
+  } else {
+    if (e >= sx * sy) return;
+    const size_t pl = (size_t)sx * sy;
+    a[e] = a[(size_t)nz * pl + e];
+    a[(size_t)(nz + 1) * pl + e] = a[pl + e];
+  }
+}
+
+// face: 1 X2max, 2 X3min, 3 X2min, 4 X3max, 5 X1min, 6 X1max; type 1 Neumann, 2 Dirichlet.
+__global__ void k_ghost_face(double* __restrict__ a, int nx, int ny, int nz, int face, int type,
+                             double v, double dedge) {
+  const int sx = nx + 2, sy = ny + 2, sz = nz + 2;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  auto at = [&](int i, int j, int k) -> double& { return a[((size_t)k * sy + j) * sx + i]; };
+  if (face == 1 || face == 3) {
+    if (e >= sx * sz) return;
+    const int i = e % sx, k = e / sx;
+    if (face == 1) at(i, ny + 1, k) = (type == 1) ? at(i, ny, k) + v * dedge : v;
+    else at(i, 0, k) = (type == 1) ? at(i, 1, k) - v * dedge : v;
+  } else if (face == 2 || face == 4) {
+    if (e >= sx * sy) return;
+    const int i = e % sx, j = e / sx;
+    if (face == 4) at(i, j, nz + 1) = (type == 1) ? at(i, j, nz) + v * dedge : v;
+    else at(i, j, 0) = (type == 1) ? at(i, j, 1) - v * dedge : v;
+  } else {
+    if (e >= sy * sz) return;
+    const int j = e % sy, k = e / sy;
+    if (face == 6) at(nx + 1, j, k) = (type == 1) ? at(nx, j, k) + v * dedge : v;
+    else at(0, j, k) = (type == 1) ? at(1, j, k) - v * dedge : v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projection step (dns/NSsolver.F90:2204-2219, :2057-2082; dns/Routines.F90:123-144).
+__global__ __launch_bounds__(kBlock) void k_psource(const double* __restrict__ U,
+                                                    const double* __restrict__ V,
+                                                    const double* __restrict__ W,
+                                                    double* __restrict__ S,
+                                                    const double* __restrict__ rho,
+                                                    const double* __restrict__ rdxc,
+                                                    const double* __restrict__ rdyc,
+                                                    const double* __restrict__ rdzc, int nx, int ny,
+                                                    int nz, double rdt) {
+  const int sx = nx + 2, sy = ny + 2, sz = nz + 2;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= sx * sy * sz) return;
+  const int i = e % sx, jk = e / sx, j = jk % sy, k = jk / sy;
+  if (i == 0 || j == 0 || k == 0 || i == nx + 1 || j == ny + 1 || k == nz + 1) {
+    S[e] = 0.0;
+    return;
+  }
+  const size_t sxy = (size_t)sx * sy;
+  const double rk = rho[k], rp = rho[k + 1], rm = rho[k - 1];
+  S[e] = ((rk * U[e] - rk * U[e - 1]) * rdxc[i] + (rk * V[e] - rk * V[e - sx]) * rdyc[j] +
+          (0.5 * (rp + rk) * W[e] - 0.5 * (rk + rm) * W[e - sxy]) * rdzc[k]) * rdt;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pgrad(const double* __restrict__ p,
+                                                  double* __restrict__ U, double* __restrict__ V,
+                                                  double* __restrict__ W,
+                                                  const double* __restrict__ rho,
+                                                  const double* __restrict__ rdxe,
+                                                  const double* __restrict__ rdye,
+                                                  const double* __restrict__ rdze, int nx, int ny,
+                                                  int nz, double dt, int wallmask) {
+  const int e0 = blockIdx.x * kBlock + threadIdx.x;
+  if (e0 >= nx * ny * nz) return;
+  const int i = e0 % nx + 1, jk = e0 / nx, j = jk % ny + 1, k = jk / ny + 1;
+  const int sx = nx + 2, sy = ny + 2;
+  const size_t sxy = (size_t)sx * sy;
+  const size_t e = (size_t)k * sxy + (size_t)j * sx + i;
+  const double rk = rho[k], rp = rho[k + 1];
+  const double pc = p[e];
+  // the last face of a direction with a pressure-Neumann upper wall is the
+  // wall itself: its normal velocity is the velocity boundary condition
+  if (!((wallmask & 1) && i == nx)) U[e] = U[e] - (p[e + 1] / rk - pc / rk) * rdxe[i] * dt;
+  if (!((wallmask & 2) && j == ny)) V[e] = V[e] - (p[e + sx] / rk - pc / rk) * rdye[j] * dt;
+  if (!((wallmask & 4) && k == nz)) W[e] = W[e] - (p[e + sxy] / rp - pc / rk) * rdze[k] * dt;
+}
+
+// Partials per block: {sum |div| vol, sum vol, max |div|, index of the max}.
+__global__ __launch_bounds__(kBlock) void k_divergence(const double* __restrict__ U,
+                                                       const double* __restrict__ V,
+                                                       const double* __restrict__ W,
+                                                       double* __restrict__ div,
+                                                       const double* __restrict__ rdxc,
+                                                       const double* __restrict__ rdyc,
+                                                       const double* __restrict__ rdzc, int nx,
+                                                       int ny, int nz, double* __restrict__ partial) {
+  const int e0 = blockIdx.x * kBlock + threadIdx.x;
+  double s[2] = {0.0, 0.0};
+  double best = -1.0;
+  long long bidx = (long long)1 << 62;
+  if (e0 < nx * ny * nz) {
+    const int i = e0 % nx + 1, jk = e0 / nx, j = jk % ny + 1, k = jk / ny + 1;
+    const int sx = nx + 2, sy = ny + 2;
+    const size_t sxy = (size_t)sx * sy;
+    const size_t e = (size_t)k * sxy + (size_t)j * sx + i;
+    const double d = (U[e] - U[e - 1]) * rdxc[i] + (V[e] - V[e - sx]) * rdyc[j] +
+                     (W[e] - W[e - sxy]) * rdzc[k];
+    if (div) div[e] = d;
+    const double vinv = rdxc[i] * rdyc[j] * rdzc[k];
+    s[0] = fabs(d) / vinv;
+    s[1] = 1.0 / vinv;
+    best = fabs(d);
+    bidx = e0;
+  }
+  // max with the smallest index on ties (the reference's first-found order)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ob = __shfl_xor(best, off, 64);
+    const long long oi = __shfl_xor(bidx, off, 64);
+    if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+  }
+  __shared__ double sb[kBlock / 64];
+  __shared__ long long si[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sb[w] = best; si[w] = bidx; }
+  double* out = partial + (size_t)blockIdx.x * 4;
+  block_reduce_store<2, true>(s, out);   // contains __syncthreads
+  if (threadIdx.x == 0) {
+    double b = sb[0];
+    long long bi = si[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+      if (sb[q] > b || (sb[q] == b && si[q] < bi)) { b = sb[q]; bi = si[q]; }
+    out[2] = b;
+    out[3] = (double)bi;
+  }
+}
+
+__global__ void k_divergence_finalize(const double* __restrict__ partial, int nb,
+                                      double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0, b = -1.0, bi = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    const double* p = partial + (size_t)q * 4;
+    s0 += p[0];
+    s1 += p[1];
+    if (p[2] > b || (p[2] == b && p[3] < bi)) { b = p[2]; bi = p[3]; }
+  }
+  out[0] = s0; out[1] = s1; out[2] = b; out[3] = bi;
+}
+
+inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---- launchers --------------------------------------------------------------
+#define LAUNCH(kern, grid, blk, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, s, __VA_ARGS__)
+
+void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
+                     const int* active, hipStream_t s) {
+  if (fp64) LAUNCH(k_rb_point<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess, active);
+  else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess, active);
+}
+void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
+                  const int* active, hipStream_t s) {
+  if (fp64) LAUNCH(k_zline<double>, ll.nblocks, 64, L, ll, parity, zero_guess, active);
+  else LAUNCH(k_zline<float>, ll.nblocks, 64, L, ll, parity, zero_guess, active);
+}
+void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                     const int* active, hipStream_t s) {
+  if (fp64) LAUNCH(k_restrict<double>, ll.nblocks, kBlock, L, X, ll, active);
+  else LAUNCH(k_restrict<float>, ll.nblocks, kBlock, L, X, ll, active);
+}
+void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                    const int* active, hipStream_t s) {
+  if (fp64) LAUNCH(k_prolong<double>, ll.nblocks, kBlock, L, X, ll, active);
+  else LAUNCH(k_prolong<float>, ll.nblocks, kBlock, L, X, ll, active);
+}
+void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, const int* active,
+                   hipStream_t s) {
+  if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll, active);
+  else LAUNCH(k_coarse<float>, ll.nblocks, kBlock, L, ll, active);
+}
+void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial,
+                         const int* active, hipStream_t s) {
+  LAUNCH(k_residual_max, ll.nblocks, kBlock, L, ll, partial, active);
+}
+void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, const int* active,
+                       hipStream_t s) {
+  LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial, active);
+}
+void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, const int* active,
+                     hipStream_t s) {
+  LAUNCH(k_defect32, ll.nblocks, kBlock, L, ll, partial, active);
+}
+void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
+                             const int* active, hipStream_t s) {
+  LAUNCH(k_correct_defect32, ll.nblocks, kBlock, L, ll, partial, active);
+}
+void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
+                     int ostride, hipStream_t s) {
+  LAUNCH(k_finalize, ll.nitems, 64, ll, partial, nvals, op_sum, out, ostride);
+}
+void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, int which,
+                         int add_fold, hipStream_t s) {
+  LAUNCH(k_from_fortran, ll.nblocks, kBlock, L, ll, ptr, which, add_fold);
+}
+void launch_wsum(const DevLevel* L, const LaunchList& ll, int which, double* partial, hipStream_t s) {
+  LAUNCH(k_wsum, ll.nblocks, kBlock, L, ll, which, partial);
+}
+void launch_shift(const DevLevel* L, const LaunchList& ll, int which, const double* c, int with_rho,
+                  hipStream_t s) {
+  LAUNCH(k_shift, ll.nblocks, kBlock, L, ll, which, c, with_rho);
+}
+void launch_to_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, hipStream_t s) {
+  LAUNCH(k_to_fortran, ll.nblocks, kBlock, L, ll, ptr);
+}
+void launch_copy_u(const DevLevel* L, const LaunchList& ll, const int* active, hipStream_t s) {
+  LAUNCH(k_copy_u, ll.nblocks, kBlock, L, ll, active);
+}
+void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s) {
+  if (fp64) LAUNCH(k_fill_zero<double>, ll.nblocks, kBlock, L, ll);
+  else LAUNCH(k_fill_zero<float>, ll.nblocks, kBlock, L, ll);
+}
+void launch_ghost_wrap(double* a, int nx, int ny, int nz, int dir, hipStream_t s) {
+  const size_t n = dir == 0 ? (size_t)(ny + 2) * (nz + 2)
+                 : dir == 1 ? (size_t)(nx + 2) * (nz + 2) : (size_t)(nx + 2) * (ny + 2);
+  LAUNCH(k_ghost_wrap, nb(n), kBlock, a, nx, ny, nz, dir);
+}
+void launch_ghost_face(double* a, int nx, int ny, int nz, int face, int type, double v,
+                       double dedge, hipStream_t s) {
+  const size_t n = (face == 1 || face == 3) ? (size_t)(nx + 2) * (nz + 2)
+                 : (face == 2 || face == 4) ? (size_t)(nx + 2) * (ny + 2)
+                                            : (size_t)(ny + 2) * (nz + 2);
+  LAUNCH(k_ghost_face, nb(n), kBlock, a, nx, ny, nz, face, type, v, dedge);
+}
+void launch_psource(const double* U, const double* V, const double* W, double* SRC, const double* rho,
+                    const double* rdxc, const double* rdyc, const double* rdzc, int nx, int ny, int nz,
+                    double rdt, hipStream_t s) {
+  LAUNCH(k_psource, nb((size_t)(nx + 2) * (ny + 2) * (nz + 2)), kBlock, U, V, W, SRC, rho, rdxc, rdyc,
+         rdzc, nx, ny, nz, rdt);
+}
+void launch_pgrad(const double* p, double* U, double* V, double* W, const double* rho,
+                  const double* rdxe, const double* rdye, const double* rdze, int nx, int ny, int nz,
+                  double dt, int wallmask, hipStream_t s) {
+  LAUNCH(k_pgrad, nb((size_t)nx * ny * nz), kBlock, p, U, V, W, rho, rdxe, rdye, rdze, nx, ny, nz, dt,
+         wallmask);
+}
+int divergence_blocks(int nx, int ny, int nz) { return (int)nb((size_t)nx * ny * nz); }
+void launch_divergence(const double* U, const double* V, const double* W, double* div,
+                       const double* rdxc, const double* rdyc, const double* rdzc, int nx, int ny,
+                       int nz, double* partial, hipStream_t s) {
+  LAUNCH(k_divergence, divergence_blocks(nx, ny, nz), kBlock, U, V, W, div, rdxc, rdyc, rdzc, nx, ny,
+         nz, partial);
+}
+void launch_divergence_finalize(const double* partial, int nblocks, double* out, hipStream_t s) {
+  LAUNCH(k_divergence_finalize, 1, 64, partial, nblocks, out);
+}
+
+}  // namespace amg

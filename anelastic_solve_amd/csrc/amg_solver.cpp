@@ -1,0 +1,702 @@
+// amg_solver.cpp -- device state, V-cycle step lists and the FP64 outer loop.
+//
+// The cycle order follows mgMK/mg_cycle.F90 (pre-smooth and restrict down
+// to the coarsest level, solve it, then prolongation-add and post-smooth up)
+// and the driver follows mgMK/mg_solver.F90 (loop of at most maxcy cycles
+// with a convergence test after each, ghosts filled on return).  Every step
+// is one launch over all instances of a set that take part in it.
+#include "amg_solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+
+namespace amg {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct Item {
+  int idx;
+  int blocks;
+};
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// d_out/h_out hold four regions of kOutRegion doubles: [0] per-instance norms
+// (4 per instance), [1] unused, [2] relmax changes (2 per instance),
+// [3] weighted sums (1 per instance).
+constexpr size_t kOutRegion = (size_t)kMaxInstances * 4;
+
+// Launch fn(ll, item_offset, block_offset) over chunks of at most kMaxItems items.
+void for_lists(const std::vector<Item>& items,
+               const std::function<void(const LaunchList&, int, int)>& fn) {
+  LaunchList ll;
+  ll.nitems = 0;
+  ll.nblocks = 0;
+  ll.prefix[0] = 0;
+  int item_off = 0, block_off = 0;
+  for (const Item& it : items) {
+    if (ll.nitems == kMaxItems) {
+      fn(ll, item_off, block_off);
+      item_off += ll.nitems;
+      block_off += ll.nblocks;
+      ll.nitems = 0;
+      ll.nblocks = 0;
+    }
+    ll.items[ll.nitems] = it.idx;
+    ll.nblocks += std::max(it.blocks, 0);
+    ll.prefix[++ll.nitems] = ll.nblocks;
+  }
+  if (ll.nitems > 0) fn(ll, item_off, block_off);
+}
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const std::string& last_error() { return g_last_error; }
+
+void* DevArena::alloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 8;
+  if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+  ptrs.push_back(p);
+  bytes += n;
+  return p;
+}
+
+void DevArena::release() {
+  for (void* p : ptrs) (void)hipFree(p);
+  ptrs.clear();
+  bytes = 0;
+}
+
+size_t cycle_bytes(const Instance& I) {
+  const double w = I.fp64 ? 8.0 : 4.0;
+  double b = 0.0;
+  const int L = (int)I.H.lv.size();
+  for (int l = 0; l + 1 < L; ++l)
+    b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + 4.25);
+  const double n0 = (double)I.H.lv[0].ncell();
+  b += I.fp64 ? 16.0 * n0 : 32.0 * n0;
+  return (size_t)b;
+}
+
+int Context::init(const amg_config& c) {
+  cfg = c;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    set_error("no HIP device visible");
+    return AMG_ENOGPU;
+  }
+  AMG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  AMG_HIP(hipHostMalloc((void**)&h_out, sizeof(double) * kOutRegion * 4, hipHostMallocDefault));
+  AMG_HIP(hipMalloc((void**)&d_out, sizeof(double) * kOutRegion * 4));
+  for (auto& e : ev) AMG_HIP(hipEventCreate(&e));
+  return AMG_OK;
+}
+
+Context::~Context() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& g : gexec)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (Instance* I : inst) delete I;
+  inst.clear();
+  if (dL) (void)hipFree(dL);
+  if (dX) (void)hipFree(dX);
+  if (d_partial) (void)hipFree(d_partial);
+  if (d_out) (void)hipFree(d_out);
+  if (h_out) (void)hipHostFree(h_out);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::remove_instance(Instance* I) {
+  auto it = std::find(inst.begin(), inst.end(), I);
+  if (it != inst.end()) inst.erase(it);
+  delete I;
+}
+
+int Context::add_instance(const amg_grid& g, Instance** out) {
+  Instance* I = new Instance();
+  std::string err;
+  int rc = build_hierarchy(cfg, g, I->H, err);
+  if (rc != AMG_OK) {
+    set_error(err);
+    delete I;
+    return rc;
+  }
+  I->id = inst.empty() ? 0 : inst.back()->id + 1;
+  I->nx = g.nx; I->ny = g.ny; I->nz = g.nz;
+  I->fp64 = (cfg.precision == AMG_FP64);
+  const int L = (int)I->H.lv.size();
+  sweep_counts(cfg, L, I->pre, I->post);
+  const size_t w = I->fp64 ? 8 : 4;
+  auto fail = [&]() {
+    set_error("device allocation failed");
+    delete I;
+    return AMG_ENOMEM;
+  };
+  // coefficient tables, volumes, density, folds
+  std::vector<DevLevel> ents(L);
+  for (int l = 0; l < L; ++l) {
+    const HostLevel& H = I->H.lv[l];
+    DevLevel& d = ents[l];
+    std::memset(&d, 0, sizeof d);
+    d.nx = H.nx; d.ny = H.ny; d.nz = H.nz; d.hx = H.nx / 2;
+    d.px = I->H.bc[0][0] == AMG_BC_PERIODIC;
+    d.py = I->H.bc[1][0] == AMG_BC_PERIODIC;
+    d.pz = I->H.bc[2][0] == AMG_BC_PERIODIC;
+    d.inst = I->id;
+    d.ncell = (int)H.ncell();
+    d.link = -1;
+    d.bcx0 = I->H.bc[0][0]; d.bcx1 = I->H.bc[0][1];
+    d.bcy0 = I->H.bc[1][0]; d.bcy1 = I->H.bc[1][1];
+    d.bcz0 = I->H.bc[2][0]; d.bcz1 = I->H.bc[2][1];
+    d.vsum = I->H.vol_sum;
+    d.vrsum = I->H.vol_rho_sum;
+    const size_t nco = H.co.size();
+    double* c64 = (double*)I->mem.alloc(nco * 8);
+    float* c32 = (float*)I->mem.alloc(nco * 4);
+    double* vol = (double*)I->mem.alloc(H.vol.size() * 8);
+    double* rho = (double*)I->mem.alloc(H.nz * 8);
+    if (!c64 || !c32 || !vol || !rho) return fail();
+    std::vector<float> co32(H.co.begin(), H.co.end());
+    AMG_HIP(hipMemcpy(c64, H.co.data(), nco * 8, hipMemcpyHostToDevice));
+    AMG_HIP(hipMemcpy(c32, co32.data(), nco * 4, hipMemcpyHostToDevice));
+    AMG_HIP(hipMemcpy(vol, H.vol.data(), H.vol.size() * 8, hipMemcpyHostToDevice));
+    AMG_HIP(hipMemcpy(rho, H.rho.data() + 1, H.nz * 8, hipMemcpyHostToDevice));
+    d.co64 = c64; d.co32 = c32; d.vol = vol; d.rho = rho;
+    if (l == 0) {
+      double* bf = (double*)I->mem.alloc(H.bcfold.size() * 8);
+      if (!bf) return fail();
+      AMG_HIP(hipMemcpy(bf, H.bcfold.data(), H.bcfold.size() * 8, hipMemcpyHostToDevice));
+      d.bcfold = bf;
+    }
+    if (l == L - 1) {
+      double* mv = (double*)I->mem.alloc(I->H.minvT.size() * 8);
+      if (!mv) return fail();
+      AMG_HIP(hipMemcpy(mv, I->H.minvT.data(), I->H.minvT.size() * 8, hipMemcpyHostToDevice));
+      d.minvT = mv;
+    }
+    // fields of the V-cycle precision
+    d.u = I->mem.alloc((size_t)d.ncell * w);
+    d.f = I->mem.alloc((size_t)d.ncell * w);
+    d.scr = I->mem.alloc((size_t)(d.ncell / 2 + 1) * w);
+    if (!d.u || !d.f || !d.scr) return fail();
+  }
+  // FP64 finest entries
+  DevLevel fA = ents[0], fB = ents[0];
+  if (I->fp64) {
+    fA.u2 = I->mem.alloc((size_t)fA.ncell * 8);   // previous iterate (relmax)
+    if (!fA.u2) return fail();
+    ents[0] = fA;
+  } else {
+    void* pA = I->mem.alloc((size_t)fA.ncell * 8);
+    void* pB = I->mem.alloc((size_t)fA.ncell * 8);
+    void* b = I->mem.alloc((size_t)fA.ncell * 8);
+    if (!pA || !pB || !b) return fail();
+    fA.u = pA; fA.u2 = pB; fA.f = b; fA.scr = nullptr;
+    fB.u = pB; fB.u2 = pA; fB.f = b; fB.scr = nullptr;
+  }
+  const int base = (int)hL.size();
+  for (int l = 0; l < L; ++l) {
+    I->lev.push_back(base + l);
+    hL.push_back(ents[l]);
+  }
+  if (I->fp64) {
+    I->fine[0] = I->fine[1] = I->lev[0];
+  } else {
+    fA.link = fB.link = I->lev[0];
+    I->fine[0] = (int)hL.size();
+    hL.push_back(fA);
+    I->fine[1] = (int)hL.size();
+    hL.push_back(fB);
+  }
+  for (int l = 0; l + 1 < L; ++l) {
+    DevXfer x;
+    x.fine = I->lev[l];
+    x.coarse = I->lev[l + 1];
+    x.fx = I->H.lv[l].fx; x.fy = I->H.lv[l].fy; x.fz = I->H.lv[l].fz;
+    I->xf.push_back((int)hX.size());
+    hX.push_back(x);
+  }
+  // projection metrics
+  {
+    const HostLevel& F = I->H.lv[0];
+    const int nx = F.nx, ny = F.ny, nz = F.nz;
+    std::vector<double> m;
+    m.insert(m.end(), F.rho.begin(), F.rho.end());
+    auto inv = [&](const std::vector<double>& a) {
+      for (double v : a) m.push_back(1.0 / v);
+    };
+    inv(F.ax.dxc); inv(F.ay.dxc); inv(F.az.dxc);
+    inv(F.ax.dxe); inv(F.ay.dxe); inv(F.az.dxe);
+    (void)nx; (void)ny; (void)nz;
+    I->d_met = (double*)I->mem.alloc(m.size() * 8);
+    if (!I->d_met) return fail();
+    AMG_HIP(hipMemcpy(I->d_met, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+  }
+  I->bytes_per_cycle = cycle_bytes(*I);
+  inst.push_back(I);
+  int rc2 = upload_tables();
+  if (rc2 != AMG_OK) return rc2;
+  *out = I;
+  return AMG_OK;
+}
+
+int Context::upload_tables() {
+  if (hL.size() > capL) {
+    if (dL) AMG_HIP(hipFree(dL));
+    capL = std::max(hL.size() * 2, (size_t)16);
+    AMG_HIP(hipMalloc((void**)&dL, capL * sizeof(DevLevel)));
+  }
+  if (hX.size() > capX || !dX) {
+    if (dX) AMG_HIP(hipFree(dX));
+    capX = std::max(hX.size() * 2, (size_t)16);
+    AMG_HIP(hipMalloc((void**)&dX, capX * sizeof(DevXfer)));
+  }
+  AMG_HIP(hipMemcpy(dL, hL.data(), hL.size() * sizeof(DevLevel), hipMemcpyHostToDevice));
+  if (!hX.empty())
+    AMG_HIP(hipMemcpy(dX, hX.data(), hX.size() * sizeof(DevXfer), hipMemcpyHostToDevice));
+  // partial buffer: enough for every instance's finest level at 4 values per block
+  size_t need = 0;
+  for (Instance* I : inst) need += (size_t)cdiv(I->H.lv[0].ncell(), kBlock) + 1;
+  need *= 4;
+  if (need > cap_partial) {
+    if (d_partial) AMG_HIP(hipFree(d_partial));
+    cap_partial = need * 2;
+    AMG_HIP(hipMalloc((void**)&d_partial, cap_partial * sizeof(double)));
+  }
+  for (auto& g : gexec)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  return AMG_OK;
+}
+
+// ---------------------------------------------------------------------------
+void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
+  if (S.empty()) return;
+  const bool zline = cfg.smoother == AMG_SMOOTHER_LJ;
+  int maxL = 0;
+  for (Instance* I : S) maxL = std::max(maxL, (int)I->lev.size());
+  const DevLevel* L = dL;
+  auto half = [&](const std::vector<Item>& items, int parity, int zg) {
+    for_lists(items, [&](const LaunchList& ll, int, int) {
+      if (zline) launch_zline(fp64, L, ll, parity, zg, nullptr, stream);
+      else launch_rb_point(fp64, L, ll, parity, zg, nullptr, stream);
+    });
+  };
+  auto blocks_smooth = [&](const DevLevel& d) {
+    return zline ? cdiv((long long)d.hx * d.ny, 64) : cdiv((long long)d.hx * d.ny * d.nz, kBlock);
+  };
+  auto smooth = [&](int dpt, bool pre, bool zero_first) {
+    int maxs = 0;
+    for (Instance* I : S)
+      if (dpt < (int)I->lev.size() - 1) maxs = std::max(maxs, pre ? I->pre[dpt] : I->post[dpt]);
+    for (int s = 0; s < maxs; ++s) {
+      std::vector<Item> items;
+      for (Instance* I : S) {
+        if (dpt >= (int)I->lev.size() - 1) continue;
+        if (s >= (pre ? I->pre[dpt] : I->post[dpt])) continue;
+        const int li = I->lev[dpt];
+        items.push_back({li, blocks_smooth(hL[li])});
+      }
+      half(items, 0, (zero_first && s == 0) ? 1 : 0);
+      half(items, 1, 0);
+    }
+  };
+  for (int dpt = 0; dpt + 1 < maxL; ++dpt) {
+    const bool zero_first = (dpt > 0) || !fp64;
+    std::vector<Item> zero;
+    for (Instance* I : S)
+      if (dpt < (int)I->lev.size() - 1 && zero_first && I->pre[dpt] == 0)
+        zero.push_back({I->lev[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
+    for_lists(zero, [&](const LaunchList& ll, int, int) { launch_fill_zero(fp64, L, ll, stream); });
+    smooth(dpt, true, zero_first);
+    std::vector<Item> items;
+    for (Instance* I : S)
+      if (dpt < (int)I->lev.size() - 1)
+        items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt + 1]].ncell, kBlock)});
+    for_lists(items, [&](const LaunchList& ll, int, int) {
+      launch_restrict(fp64, L, dX, ll, nullptr, stream);
+    });
+  }
+  {
+    std::vector<Item> items;
+    for (Instance* I : S) {
+      const int li = I->lev.back();
+      items.push_back({li, cdiv(hL[li].ncell, kBlock)});
+    }
+    for_lists(items, [&](const LaunchList& ll, int, int) { launch_coarse(fp64, L, ll, nullptr, stream); });
+  }
+  for (int dpt = maxL - 2; dpt >= 0; --dpt) {
+    std::vector<Item> items;
+    for (Instance* I : S)
+      if (dpt < (int)I->lev.size() - 1)
+        items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
+    for_lists(items, [&](const LaunchList& ll, int, int) {
+      launch_prolong(fp64, L, dX, ll, nullptr, stream);
+    });
+    smooth(dpt, false, false);
+  }
+}
+
+namespace {
+std::vector<Item> fine_items(const Context& C, const std::vector<Instance*>& S, bool current) {
+  std::vector<Item> items;
+  for (Instance* I : S) {
+    const int li = current ? I->fine[I->cur] : I->fine[0];
+    items.push_back({li, cdiv(C.hL[li].ncell, kBlock)});
+  }
+  return items;
+}
+}  // namespace
+
+// Launches one cycle on every instance of S plus the norm kernels and the
+// device->host copy of the per-instance results (h_out, nvals per item).
+int Context::cycle_and_norms(const std::vector<Instance*>& S) {
+  if (S.empty()) return AMG_OK;
+  const bool fp64 = S[0]->fp64;
+  const bool relmax = fp64 && cfg.report_relmax;
+  if (relmax) {
+    auto items = fine_items(*this, S, true);
+    for_lists(items, [&](const LaunchList& ll, int, int) { launch_copy_u(dL, ll, nullptr, stream); });
+  }
+  vcycle(S, fp64);
+  auto items = fine_items(*this, S, true);
+  const int nv = 4;
+  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
+    double* part = d_partial + (size_t)boff * nv;
+    if (fp64) {
+      launch_residual_max(dL, ll, part, nullptr, stream);
+      launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
+      if (relmax) {
+        launch_change_max(dL, ll, part, nullptr, stream);
+        // stored at value slots 2,3
+        launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
+      }
+    } else {
+      launch_correct_defect32(dL, ll, part, nullptr, stream);
+      launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream);
+    }
+  });
+  (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
+  if (relmax)
+    (void)hipMemcpyAsync(h_out + kOutRegion * 2, d_out + kOutRegion * 2, sizeof(double) * S.size() * 2,
+                         hipMemcpyDeviceToHost, stream);
+  return AMG_OK;
+}
+
+// Reads the results of cycle_and_norms (after the stream is synchronised).
+void Context::post_norms(const std::vector<Instance*>& S) {
+  const int nv = 4;
+  for (size_t q = 0; q < S.size(); ++q) {
+    Instance* I = S[q];
+    I->iters += 1;
+    const double* o = h_out + q * nv;
+    const double rmax = o[0];
+    double relmax = 0.0;
+    if (I->fp64) {
+      if (cfg.report_relmax) {
+        const double* c = h_out + kOutRegion * 2 + q * 2;
+        relmax = c[1] > 0.0 ? c[0] / c[1] : 0.0;
+      } else {
+        relmax = std::nan("");
+      }
+    } else {
+      relmax = o[2] > 0.0 ? o[1] / o[2] : 0.0;
+      I->cur ^= 1;
+    }
+    I->relres = I->bnorm > 0.0 ? rmax / I->bnorm : rmax;
+    I->relmax = relmax;
+    if (cfg.verbose) {
+      if (std::isnan(relmax))
+        std::fprintf(stderr, "       Iteration: %5d  relres= %12.5e\n", I->iters, I->relres);
+      else
+        std::fprintf(stderr, "       Iteration: %5d  RESIDUAL= %22.15e  relres= %12.5e\n", I->iters,
+                     relmax, I->relres);
+    }
+    if (!std::isfinite(rmax)) {
+      I->status = AMG_ENAN;
+      I->done = true;
+      continue;
+    }
+    const double crit = (cfg.stop_rule == AMG_STOP_RELMAX) ? relmax : I->relres;
+    if (crit <= cfg.tol) {
+      I->converged = true;
+      I->done = true;
+    } else if (I->iters >= cfg.maxcy) {
+      I->done = true;
+    }
+  }
+}
+
+int Context::prepare(const std::vector<Instance*>& S) {
+  const int nv = 4;
+  PtrList rhs{}, phi{};
+  std::vector<Item> items;
+  for (size_t q = 0; q < S.size(); ++q) {
+    Instance* I = S[q];
+    I->cur = 0;
+    I->iters = 0;
+    I->converged = false;
+    I->done = false;
+    I->status = AMG_OK;
+    items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, kBlock)});
+  }
+  // Fortran arrays -> chequer FP64 (b with the boundary fold)
+  for_lists(items, [&](const LaunchList& ll, int ioff, int) {
+    for (int t = 0; t < ll.nitems; ++t) {
+      rhs.p[t] = const_cast<double*>(S[ioff + t]->d_rhs);
+      phi.p[t] = S[ioff + t]->d_phi;
+    }
+    launch_from_fortran(dL, ll, rhs, 1, 1, stream);
+    launch_from_fortran(dL, ll, phi, 0, 0, stream);
+  });
+  // singular: remove the incompatible part of b (sum V b = 0)
+  std::vector<Item> sing;
+  for (Instance* I : S)
+    if (I->H.singular) sing.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, kBlock)});
+  for_lists(sing, [&](const LaunchList& ll, int ioff, int boff) {
+    double* part = d_partial + (size_t)boff * nv;
+    launch_wsum(dL, ll, 1, part, stream);
+    launch_finalize(ll, part, 1, 1, d_out + kOutRegion * 3 + ioff, 1, stream);
+    launch_shift(dL, ll, 1, d_out + kOutRegion * 3 + ioff, 0, stream);
+  });
+  // initial residual (mixed: also the FP32 defect of the first cycle)
+  const bool fp64 = S[0]->fp64;
+  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
+    double* part = d_partial + (size_t)boff * nv;
+    if (fp64) launch_residual_max(dL, ll, part, nullptr, stream);
+    else launch_defect32(dL, ll, part, nullptr, stream);
+    launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
+  });
+  AMG_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream));
+  AMG_HIP(hipStreamSynchronize(stream));
+  for (size_t q = 0; q < S.size(); ++q) {
+    Instance* I = S[q];
+    const double r = h_out[q * nv], b = h_out[q * nv + 1];
+    I->bnorm = b;
+    I->res0 = b > 0.0 ? r / b : r;
+    I->relres = I->res0;
+    I->relmax = 0.0;
+    if (!std::isfinite(r) || !std::isfinite(b)) {
+      I->status = AMG_ENAN;
+      I->done = true;
+    } else if (cfg.stop_rule == AMG_STOP_RESIDUAL && (b == 0.0 || I->res0 <= cfg.tol)) {
+      I->converged = true;
+      I->done = true;
+    }
+  }
+  return AMG_OK;
+}
+
+int Context::finish(const std::vector<Instance*>& S) {
+  const int nv = 4;
+  std::vector<Item> cur, sing;
+  for (Instance* I : S) {
+    const int li = I->fine[I->cur];
+    cur.push_back({li, cdiv(hL[li].ncell, kBlock)});
+    if (I->H.singular) sing.push_back({li, cdiv(hL[li].ncell, kBlock)});
+  }
+  // null-space removal: p -= (sum V p / sum V rho) rho
+  for_lists(sing, [&](const LaunchList& ll, int ioff, int boff) {
+    double* part = d_partial + (size_t)boff * nv;
+    launch_wsum(dL, ll, 0, part, stream);
+    launch_finalize(ll, part, 1, 1, d_out + kOutRegion * 3 + ioff, 1, stream);
+    launch_shift(dL, ll, 0, d_out + kOutRegion * 3 + ioff, 1, stream);
+  });
+  PtrList phi{};
+  for_lists(cur, [&](const LaunchList& ll, int ioff, int) {
+    for (int t = 0; t < ll.nitems; ++t) phi.p[t] = S[ioff + t]->d_phi;
+    launch_to_fortran(dL, ll, phi, stream);
+  });
+  // ghost layers as mg_bdry(ngrid, phif, .true., .true.) (mgMK/mg_solver.F90:139)
+  for (Instance* I : S) {
+    const HostLevel& F = I->H.lv[0];
+    double* a = I->d_phi;
+    for (int d = 0; d < 3; ++d) launch_ghost_wrap(a, F.nx, F.ny, F.nz, d, stream);
+    const int* t = cfg.tbc;
+    const double* v = cfg.vbc;
+    const double edge[7] = {0.0, F.ay.dxc[F.ny], F.az.dxc[1], F.ay.dxc[1], F.az.dxc[F.nz],
+                            F.ax.dxc[1], F.ax.dxc[F.nx]};
+    for (int face = 1; face <= 6; ++face)
+      if (t[face - 1] != AMG_BC_PERIODIC)
+        launch_ghost_face(a, F.nx, F.ny, F.nz, face, t[face - 1], v[face - 1], edge[face], stream);
+  }
+  return AMG_OK;
+}
+
+int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
+  if (S.empty()) return AMG_OK;
+  AMG_HIP(hipEventRecord(ev[2], stream));
+  int rc = prepare(S);
+  if (rc != AMG_OK) return rc;
+  AMG_HIP(hipEventRecord(ev[0], stream));
+  const bool single = S.size() == 1 && cfg.use_graph;
+  double bytes = 0.0;
+  for (int it = 0; it < cfg.maxcy; ++it) {
+    std::vector<Instance*> A;
+    for (Instance* I : S)
+      if (!I->done) A.push_back(I);
+    if (A.empty()) break;
+    if (single && graph_ok) {
+      Instance* I = A[0];
+      const int gi = I->fp64 ? 0 : 1 + I->cur;
+      if (!gexec[gi]) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          cycle_and_norms(A);
+          ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
+        }
+        if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+          graph_ok = false;
+          gexec[gi] = nullptr;
+          (void)hipGetLastError();
+        }
+      }
+      if (gexec[gi]) AMG_HIP(hipGraphLaunch(gexec[gi], stream));
+      else cycle_and_norms(A);
+    } else {
+      cycle_and_norms(A);
+    }
+    AMG_HIP(hipStreamSynchronize(stream));
+    post_norms(A);
+    for (Instance* I : A) bytes += (double)I->bytes_per_cycle;
+  }
+  AMG_HIP(hipEventRecord(ev[1], stream));
+  rc = finish(S);
+  if (rc != AMG_OK) return rc;
+  AMG_HIP(hipEventRecord(ev[3], stream));
+  AMG_HIP(hipStreamSynchronize(stream));
+  AMG_HIP(hipGetLastError());
+  int status = AMG_OK;
+  for (Instance* I : S) {
+    if (I->status != AMG_OK) status = I->status;
+    else if (!I->converged) {
+      I->status = AMG_ENOTCONV;
+      if (status == AMG_OK) status = AMG_ENOTCONV;
+    }
+  }
+  if (st) {
+    float ms = 0.f;
+    std::memset(st, 0, sizeof *st);
+    if (hipEventElapsedTime(&ms, ev[2], ev[3]) == hipSuccess) st->ms_total = ms;
+    if (hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) st->ms_cycles = ms;
+    st->bytes_cycles = bytes;
+    Instance* I = S[0];
+    st->iters = I->converged ? I->iters : cfg.maxcy + 1;
+    st->converged = I->converged;
+    st->relres = I->relres;
+    st->relmax = I->relmax;
+    st->res0 = I->res0;
+    st->bnorm = I->bnorm;
+    st->nlevels = (int)I->H.lv.size();
+    for (int l = 0; l < st->nlevels && l < AMG_MAX_LEVELS; ++l) {
+      st->level_n[l][0] = I->H.lv[l].nx;
+      st->level_n[l][1] = I->H.lv[l].ny;
+      st->level_n[l][2] = I->H.lv[l].nz;
+    }
+  }
+  return status;
+}
+
+int Context::run_kernel(Instance* I, int which, int level, int reps, double* bytes) {
+  const int L = (int)I->lev.size();
+  if (level < 0 || level >= L || reps < 0) {
+    set_error("amg_run_kernel: bad level or reps");
+    return AMG_EINVAL;
+  }
+  const bool fp64 = I->fp64;
+  const double w = fp64 ? 8.0 : 4.0;
+  const DevLevel& d = hL[I->lev[level]];
+  const bool zline = cfg.smoother == AMG_SMOOTHER_LJ;
+  std::vector<Item> items;
+  double b = 0.0;
+  if (which == 0 || which == 4 || which == 6) {
+    items.push_back({I->lev[level], zline ? cdiv((long long)d.hx * d.ny, 64)
+                                          : cdiv((long long)d.hx * d.ny * d.nz, kBlock)});
+    b = 1.5 * w * d.ncell;
+  } else if (which == 1 || which == 2) {
+    if (level >= L - 1) {
+      set_error("amg_run_kernel: no coarser level");
+      return AMG_EINVAL;
+    }
+    const int nc = which == 1 ? hL[I->lev[level + 1]].ncell : d.ncell;
+    items.push_back({I->xf[level], cdiv(nc, kBlock)});
+    b = (2.0 + 0.125) * w * d.ncell;
+  } else if (which == 3) {
+    items.push_back({I->fine[I->cur], cdiv(hL[I->fine[I->cur]].ncell, kBlock)});
+    b = fp64 ? 16.0 * d.ncell : 32.0 * d.ncell;
+  } else if (which == 5) {
+    if (level != L - 1) {
+      set_error("amg_run_kernel: the coarse solve runs on the coarsest level");
+      return AMG_EINVAL;
+    }
+    items.push_back({I->lev[level], cdiv(d.ncell, kBlock)});
+    b = 2.0 * w * d.ncell;
+  } else {
+    set_error("amg_run_kernel: unknown kernel");
+    return AMG_EINVAL;
+  }
+  for (int r = 0; r < reps; ++r) {
+    for_lists(items, [&](const LaunchList& ll, int, int) {
+      if (which == 0 || which == 4 || which == 6) {
+        const int par = which == 4 ? 1 : 0, zg = which == 6 ? 1 : 0;
+        if (zline) launch_zline(fp64, dL, ll, par, zg, nullptr, stream);
+        else launch_rb_point(fp64, dL, ll, par, zg, nullptr, stream);
+      } else if (which == 1) {
+        launch_restrict(fp64, dL, dX, ll, nullptr, stream);
+      } else if (which == 2) {
+        launch_prolong(fp64, dL, dX, ll, nullptr, stream);
+      } else if (which == 5) {
+        launch_coarse(fp64, dL, ll, nullptr, stream);
+      } else if (fp64) {
+        launch_residual_max(dL, ll, d_partial, nullptr, stream);
+      } else {
+        launch_correct_defect32(dL, ll, d_partial, nullptr, stream);
+      }
+    });
+  }
+  AMG_HIP(hipGetLastError());
+  if (bytes) *bytes = b;
+  return AMG_OK;
+}
+
+int Context::level_field(Instance* I, int level, int field, int dir, double* buf) {
+  const int L = (int)I->lev.size();
+  if (level < -1 || level >= L || (field != 0 && field != 1) || !buf) {
+    set_error("amg_level_field: bad argument");
+    return AMG_EINVAL;
+  }
+  const bool fine64 = level == -1;
+  const DevLevel& d = hL[fine64 ? I->fine[I->cur] : I->lev[level]];
+  const bool dbl = fine64 || I->fp64;
+  const size_t n = (size_t)d.ncell;
+  void* dev = field == 0 ? d.u : d.f;
+  std::vector<double> h64;
+  std::vector<float> h32;
+  if (dbl) h64.resize(n); else h32.resize(n);
+  void* hp = dbl ? (void*)h64.data() : (void*)h32.data();
+  const size_t nb = n * (dbl ? 8 : 4);
+  AMG_HIP(hipStreamSynchronize(stream));
+  if (dir == 0) AMG_HIP(hipMemcpy(hp, dev, nb, hipMemcpyDeviceToHost));
+  for (int k = 0; k < d.nz; ++k)
+    for (int j = 0; j < d.ny; ++j)
+      for (int i = 0; i < d.nx; ++i) {
+        const size_t nat = ((size_t)k * d.ny + j) * d.nx + i;
+        const size_t ch = (size_t)chequer_index(i, j, k, d.nx, d.ny);
+        if (dir == 0) buf[nat] = dbl ? h64[ch] : (double)h32[ch];
+        else if (dbl) h64[ch] = buf[nat];
+        else h32[ch] = (float)buf[nat];
+      }
+  if (dir == 1) AMG_HIP(hipMemcpy(dev, hp, nb, hipMemcpyHostToDevice));
+  return AMG_OK;
+}
+
+}  // namespace amg

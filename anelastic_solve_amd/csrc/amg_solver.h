@@ -1,0 +1,101 @@
+// amg_solver.h -- host orchestration of the multigrid pressure solve on one
+// GPU: device state per instance, V-cycle step lists over a set of
+// instances (one launch per level step), the FP64 outer loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "amg.h"
+#include "amg_hierarchy.h"
+#include "amg_internal.h"
+
+namespace amg {
+
+void set_error(const std::string& msg);
+const std::string& last_error();
+
+constexpr int kMaxInstances = 1024;
+
+#define AMG_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));          \
+      return AMG_EHIP;                                                       \
+    }                                                                        \
+  } while (0)
+
+// Owns device allocations.
+struct DevArena {
+  std::vector<void*> ptrs;
+  size_t bytes = 0;
+  void* alloc(size_t n);
+  void release();
+  ~DevArena() { release(); }
+};
+
+// One Poisson problem resident on the device.
+struct Instance {
+  int id = 0;
+  Hierarchy H;
+  int nx = 0, ny = 0, nz = 0;
+  bool fp64 = false;             // V-cycle precision (true: plain FP64)
+  std::vector<int> lev;          // DevLevel index per level, finest first
+  std::vector<int> xf;           // DevXfer index per transition
+  int fine[2] = {-1, -1};        // FP64 finest entries (u = p_cur for cur = 0/1)
+  int cur = 0;                   // which fine entry holds the current pressure
+  std::vector<int> pre, post;
+  DevArena mem;
+  // projection metrics (ghost-inclusive): rho, rdxc, rdyc, rdzc, rdxe, rdye, rdze
+  double* d_met = nullptr;
+  // per-solve state
+  bool converged = false, done = false;
+  int iters = 0;
+  double bnorm = 0.0, relres = 0.0, relmax = 0.0, res0 = 0.0;
+  int status = AMG_OK;
+  size_t bytes_per_cycle = 0;
+  double* d_phi = nullptr;       // caller arrays of the current solve
+  const double* d_rhs = nullptr;
+  size_t device_bytes() const { return mem.bytes; }
+};
+
+// A set of instances sharing one stream and one descriptor table.
+struct Context {
+  amg_config cfg{};
+  hipStream_t stream = nullptr;
+  std::vector<Instance*> inst;
+  std::vector<DevLevel> hL;
+  std::vector<DevXfer> hX;
+  DevLevel* dL = nullptr;
+  DevXfer* dX = nullptr;
+  size_t capL = 0, capX = 0;
+  double* d_partial = nullptr;   // block partials
+  size_t cap_partial = 0;
+  double* d_out = nullptr;       // per-item results
+  double* h_out = nullptr;       // pinned mirror
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // single-instance graphs: [0] fp64 cycle, [1]/[2] mixed cycle for cur = 0/1
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  bool graph_ok = true;
+
+  int init(const amg_config& c);
+  ~Context();
+  int add_instance(const amg_grid& g, Instance** out);   // builds + allocates
+  int upload_tables();
+  int solve(const std::vector<Instance*>& S, amg_stats* st);
+  // pieces
+  void vcycle(const std::vector<Instance*>& S, bool fp64);
+  int prepare(const std::vector<Instance*>& S);
+  int finish(const std::vector<Instance*>& S);
+  int cycle_and_norms(const std::vector<Instance*>& S);
+  void post_norms(const std::vector<Instance*>& S);
+  int run_kernel(Instance* I, int which, int level, int reps, double* bytes);
+  int level_field(Instance* I, int level, int field, int dir, double* buf);
+  void remove_instance(Instance* I);
+};
+
+size_t cycle_bytes(const Instance& I);
+
+}  // namespace amg

@@ -1,0 +1,383 @@
+"""Pins of the oracle against mathematics (CPU only).
+
+The reference stores no golden vectors for this path and its Fortran needs
+an MPI library to build, so every oracle function is pinned here by
+closed forms, invariants, brute force on tiny grids, or agreement of two
+independent restatements -- each chosen so that a dropped term, a wrong
+sign or index, or a transposed operand fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import direct, ops, vcycle as VC
+from oracle import grid as G
+
+
+def _levels(n, L=(2 * math.pi, 2 * math.pi, 1.0), beta=0.0, tbc=(0, 1, 0, 1, 0, 0), rho=None,
+            factors=None):
+    nx, ny, nz = n
+    ax = G.uniform_axis(nx, L[0])
+    ay = G.uniform_axis(ny, L[1])
+    az = G.tanh_axis(nz, L[2], beta)
+    r = np.ones(nz + 2) if rho is None else rho
+    fac = G.auto_factors(nx, ny, nz) if factors is None else factors
+    return G.hierarchy(ax, ay, az, r, list(tbc), fac), fac, (ax, ay, az)
+
+
+# ---------------------------------------------------------------- grid
+def test_axis_metrics():
+    a = G.tanh_axis(32, 1.0, 2.0)
+    n = a["n"]
+    assert abs(a["dxc"][1:n + 1].sum() - 1.0) < 1e-14
+    assert np.all(a["dxc"][1:n + 1] > 0)
+    # centre distances and the ghost fix-ups of utils/loadGrid.F90:79-90
+    assert np.allclose(a["dxe"][1:n], a["xc"][2:n + 1] - a["xc"][1:n], rtol=0, atol=1e-15)
+    assert a["dxe"][0] == a["dxe"][1] and a["dxc"][0] == a["dxc"][1]
+    # mid-plane clustering: spacing ratio tends to cosh(b)^2 as n grows
+    b = G.tanh_axis(256, 1.0, 2.0)
+    r = b["dxc"][1:257].max() / b["dxc"][1:257].min()
+    assert 0.9 * math.cosh(2.0) ** 2 < r < math.cosh(2.0) ** 2
+    assert b["dxc"][128] < b["dxc"][1]
+    u = G.uniform_axis(8, 2.0)
+    assert np.allclose(u["dxc"], 0.25) and np.allclose(u["dxe"], 0.25)
+    c = G.coarsen_axis(u)
+    assert c["n"] == 4 and np.allclose(c["dxc"], 0.5) and np.allclose(c["dxe"], 0.5)
+
+
+def test_product_grid_equals_oracle_grid():
+    from anelastic_solve_amd import grid as PG
+    for beta in (0.0, 1.3, 2.5):
+        p = PG.tanh_axis(24, 1.0, beta)
+        o = G.tanh_axis(24, 1.0, beta)
+        for k in ("xe", "xc", "dxe", "dxc"):
+            assert np.array_equal(getattr(p, k), o[k]), (beta, k)
+
+
+# ---------------------------------------------------------------- operator
+def test_fourier_eigenvalues():
+    """Periodic x/y modes are eigenfunctions with -4/dx^2 sin^2(pi m/n) (+ y part)."""
+    levs, _, (ax, ay, az) = _levels((16, 12, 8))
+    lev = levs[0]
+    nx, ny, nz = lev["n"]
+    dx, dy = 2 * math.pi / nx, 2 * math.pi / ny
+    i = np.arange(nx)[None, None, :]
+    j = np.arange(ny)[None, :, None]
+    for mx, my in ((1, 0), (3, 2), (8, 6), (5, 1)):
+        u = np.cos(2 * math.pi * mx * i / nx) * np.cos(2 * math.pi * my * j / ny) * np.ones((nz, 1, 1))
+        lam = -4 / dx ** 2 * math.sin(math.pi * mx / nx) ** 2 - 4 / dy ** 2 * math.sin(math.pi * my / ny) ** 2
+        assert np.abs(ops.apply_A(u, lev) - lam * u).max() < 1e-10 * abs(lam)
+
+
+def test_psource_pgrad_compose_to_operator():
+    """D(rho G(p/rho)) == A p with stretched z and a non-uniform density.
+
+    Ties three restatements (Psource, Pgrad, the coefficient tables) together:
+    a wrong density factor, metric index or sign in any of them fails.
+    """
+    n = 12
+    z = np.linspace(-0.05, 1.05, n + 2)
+    rho = 1.2 - 0.4 * z + 0.1 * z ** 2
+    levs, _, (ax, ay, az) = _levels((n, 10, n), beta=1.7, rho=rho)
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal((n + 2, 12, n + 2))
+    tbc = [0, 1, 0, 1, 0, 0]
+    ops.fill_ghosts(p, tbc, [-1, 0, -1, 0, -1, -1], ax, ay, az)
+    # zero-flux walls: the ghost carries p/rho continuity
+    p[0] = p[1] * rho[0] / rho[1]
+    p[n + 1] = p[n] * rho[n + 1] / rho[n]
+    Z = np.zeros_like(p)
+    U, V, W = ops.pgrad(p, Z, Z, Z, rho, ax, ay, az, 1.0)
+    ops.wrap_periodic(U, (1, 1, 0)); ops.wrap_periodic(V, (1, 1, 0))
+    W[0] = 0.0
+    W[n] = 0.0
+    s = ops.psource(U, V, W, rho, ax, ay, az, 1.0)
+    Ap = ops.apply_A(p[1:-1, 1:-1, 1:-1], levs[0])
+    assert np.abs(-s[1:-1, 1:-1, 1:-1] - Ap).max() <= 1e-12 * np.abs(Ap).max()
+
+
+def test_table_form_equals_ghost_form():
+    """Coefficient tables (walls folded) == the reference's ghosted seven-point sum."""
+    levs, _, (ax, ay, az) = _levels((10, 8, 12), beta=1.0, tbc=(2, 1, 1, 2, 0, 0))
+    tbc = [2, 1, 1, 2, 0, 0]
+    rng = np.random.default_rng(2)
+    p = rng.standard_normal((14, 10, 12))
+    ops.fill_ghosts(p, tbc, [0.0] * 6, ax, ay, az)
+    a = ops.apply_A(p[1:-1, 1:-1, 1:-1], levs[0])
+    b = ops.apply_A_ghosted(p, ax, ay, az, np.ones(14))
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+
+
+def test_bc_fold_matches_ghosts():
+    """Inhomogeneous Neumann/Dirichlet ghosts == homogeneous operator + folded RHS."""
+    tbc = [2, 1, 1, 2, 2, 1]
+    vbc = [0.3, -0.7, 1.1, 2.0, -0.4, 0.9]
+    levs, _, (ax, ay, az) = _levels((8, 10, 12), beta=1.2, tbc=tbc, L=(1.0, 2.0, 1.0))
+    rng = np.random.default_rng(3)
+    p = rng.standard_normal((14, 12, 10))
+    ops.fill_ghosts(p, tbc, vbc, ax, ay, az)
+    ghost_form = ops.apply_A_ghosted(p, ax, ay, az, np.ones(14))
+    zero = np.zeros((12, 10, 8))
+    table = ops.apply_A(p[1:-1, 1:-1, 1:-1], levs[0]) - ops.fold_bc_rhs(zero, levs[0], tbc, vbc)
+    assert np.abs(ghost_form - table).max() <= 1e-12 * np.abs(table).max()
+
+
+# ---------------------------------------------------------------- direct solvers
+def test_fft_and_sparse_agree():
+    z = np.linspace(-0.05, 1.05, 14)
+    rho = 1.0 - 0.2 * z
+    levs, _, _ = _levels((16, 8, 12), beta=2.0, rho=rho)
+    rng = np.random.default_rng(4)
+    b = ops.compat_project(rng.standard_normal((12, 8, 16)), levs[0])
+    x1 = direct.solve_fft(levs[0], b, True)
+    x2 = direct.solve_sparse(levs[0], b, True)
+    assert np.abs(x1 - x2).max() <= 1e-11 * np.abs(x1).max()
+    assert np.abs(ops.apply_A(x1, levs[0]) - b).max() <= 1e-11 * np.abs(b).max()
+    # the null-space convention: integral of p is zero
+    assert abs((ops.volumes(levs[0]) * x1).sum()) < 1e-12
+
+
+def test_dense_brute_force_tiny():
+    """4x2x3 grid: sparse solve == numpy dense solve of the brute-force matrix."""
+    levs, _, _ = _levels((4, 2, 3), tbc=(2, 1, 1, 2, 2, 2), factors=[])
+    lev = levs[0]
+    N = 24
+    A = np.zeros((N, N))
+    for c in range(N):
+        e = np.zeros(N)
+        e[c] = 1.0
+        A[:, c] = ops.apply_A(e.reshape(3, 2, 4), lev).ravel()
+    b = np.arange(N, dtype=float) - 7.0
+    x = direct.solve_sparse(lev, b.reshape(3, 2, 4), False)
+    assert np.allclose(x.ravel(), np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+
+
+def _kat_levels(n, tbc, L=1.0):
+    ax = G.uniform_axis(n, L)
+    ay = G.uniform_axis(4, L)
+    az = G.uniform_axis(4, L)
+    return G.hierarchy(ax, ay, az, np.ones(6), list(tbc), [])[0], ax
+
+
+def test_closed_form_dirichlet_quadratic():
+    """mgMK/test/test.F90:150-189 (dirichletBCs): src = a1 = 2, p = a1 x^2/2 + b1 x + c1,
+    Dirichlet values 1, 2 at the ghost centres."""
+    tbc = [0, 0, 0, 0, 2, 2]
+    vbc = [-1, -1, -1, -1, 1.0, 2.0]
+    lev, ax = _kat_levels(16, tbc)
+    b = ops.fold_bc_rhs(np.full((4, 4, 16), 2.0), lev, tbc, vbc)
+    x = direct.solve_sparse(lev, b, False)
+    x0, x1 = ax["xc"][0], ax["xc"][17]
+    b1 = (2.0 - 1.0 - (x1 ** 2 - x0 ** 2)) / (x1 - x0)
+    c1 = 1.0 - x0 ** 2 - b1 * x0
+    exact = ax["xc"][1:17] ** 2 + b1 * ax["xc"][1:17] + c1
+    assert np.abs(x - exact[None, None, :]).max() < 1e-12
+
+
+def test_closed_form_linear_and_neumann():
+    """linearpgrad (:191-227): src=0 between Dirichlet 1 and 2 is linear;
+    1dirichlet1NeumannBC (:229-268): src=2, Dirichlet 1 at X1min, dp/dx = 0.5 at
+    X1max on a uniform grid: p = x^2 + b1 x + c1 with p'(L) = 0.5."""
+    tbc = [0, 0, 0, 0, 2, 2]
+    lev, ax = _kat_levels(12, tbc)
+    b = ops.fold_bc_rhs(np.zeros((4, 4, 12)), lev, tbc, [-1, -1, -1, -1, 1.0, 2.0])
+    x = direct.solve_sparse(lev, b, False)
+    xc = ax["xc"]
+    lin = 1.0 + (xc[1:13] - xc[0]) / (xc[13] - xc[0])
+    assert np.abs(x - lin[None, None, :]).max() < 1e-12
+    tbc2 = [0, 0, 0, 0, 2, 1]
+    lev2, ax2 = _kat_levels(12, tbc2)
+    g = 0.5
+    b2 = ops.fold_bc_rhs(np.full((4, 4, 12), 2.0), lev2, tbc2, [-1, -1, -1, -1, 1.0, g])
+    x2 = direct.solve_sparse(lev2, b2, False)
+    # discrete Neumann: (p_{n+1} - p_n)/dx = g with p_{n+1} - p_n = g*dxc(n)
+    xc = ax2["xc"]
+    L = ax2["xe"][12]
+    b1 = g - 2.0 * L    # p'(x) = 2x + b1, p'(L) = g at the wall face
+    c1 = 1.0 - xc[0] ** 2 - b1 * xc[0]
+    exact = xc[1:13] ** 2 + b1 * xc[1:13] + c1
+    assert np.abs(x2 - exact[None, None, :]).max() < 1e-10
+
+
+def test_closed_form_cfdhw_linear_in_y():
+    """cfdhw (mgMK/test/test.F90:347-380): Dirichlet in y, Neumann in z, src = 0:
+    the solution is linear in y whatever the initial guess."""
+    tbc = [2, 1, 2, 1, 0, 0]
+    ax = G.uniform_axis(8, 1.0)
+    ay = G.uniform_axis(10, 1.0)
+    az = G.uniform_axis(8, 1.0)
+    levs = G.hierarchy(ax, ay, az, np.ones(10), tbc, G.auto_factors(8, 10, 8))
+    vbc = [3.0, 0.0, -1.0, 0.0, -1, -1]
+    b = ops.fold_bc_rhs(np.zeros((8, 10, 8)), levs[0], tbc, vbc)
+    x = direct.solve_sparse(levs[0], b, False)
+    yc = ay["xc"]
+    lin = -1.0 + 4.0 * (yc[1:11] - yc[0]) / (yc[11] - yc[0])
+    assert np.abs(x - lin[None, :, None]).max() < 1e-12
+
+
+# ---------------------------------------------------------------- smoothers
+def _literal_rb(u, f, lev, parity):
+    """Pure-Python point RB half-sweep in the reference's loop order
+    (mgMK/mg_relax.F90:149-167: k, j outer, i inner stepping by 2)."""
+    nz, ny, nx = u.shape
+    wxm, wxp, dx = lev["cx"][:3]
+    wym, wyp, dy = lev["cy"][:3]
+    wzm, wzp, dz = lev["cz"][:3]
+    px, py, pz = lev["per"]
+    def at(k, j, i):
+        if px: i %= nx
+        if py: j %= ny
+        if pz: k %= nz
+        if 0 <= i < nx and 0 <= j < ny and 0 <= k < nz:
+            return u[k, j, i]
+        return 0.0
+    for k in range(nz):
+        for j in range(ny):
+            i0 = (parity + j + k) % 2
+            for i in range(i0, nx, 2):
+                s = (wxm[i] * at(k, j, i - 1) + wxp[i] * at(k, j, i + 1) + wym[j] * at(k, j - 1, i)
+                     + wyp[j] * at(k, j + 1, i) + wzm[k] * at(k - 1, j, i) + wzp[k] * at(k + 1, j, i))
+                u[k, j, i] = (f[k, j, i] - s) / (dx[i] + dy[j] + dz[k])
+    return u
+
+
+@pytest.mark.parametrize("tbc", [(0, 1, 0, 1, 0, 0), (2, 1, 1, 2, 2, 1), (0, 0, 0, 0, 0, 0)])
+def test_rb_sweep_equals_literal_loop(tbc):
+    levs, _, _ = _levels((6, 4, 6), beta=1.0, tbc=tbc, factors=[])
+    lev = levs[0]
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((6, 4, 6))
+    f = rng.standard_normal((6, 4, 6))
+    for parity in (0, 1):
+        a = VC.rb_half_sweep(u.copy(), f, lev, parity)
+        b = _literal_rb(u.copy(), f, lev, parity)
+        assert np.abs(a - b).max() < 1e-13
+
+
+def test_rb_sweep_invariants():
+    levs, _, _ = _levels((8, 8, 8), beta=1.5)
+    lev = levs[0]
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((8, 8, 8))
+    f = ops.apply_A(x, lev)
+    # the exact solution is a fixed point
+    assert np.abs(VC.smooth(x.copy(), f, lev, 1, VC.RBPGS) - x).max() < 1e-12
+    assert np.abs(VC.smooth(x.copy(), f, lev, 1, VC.LJ) - x).max() < 1e-12
+    # after a colour's pass its residual vanishes
+    u = rng.standard_normal((8, 8, 8))
+    for line, hs in ((False, VC.rb_half_sweep), (True, VC.zline_half_sweep)):
+        v = hs(u.copy(), f, lev, 0)
+        r = VC.residual(v, f, lev)
+        m = VC.colour_mask(u.shape, 0, line)
+        assert np.abs(r[m]).max() < 1e-10 * np.abs(f).max()
+        assert np.abs(r[~m]).max() > 1e-3
+
+
+def test_zline_equals_column_loop():
+    """Vectorised z-line sweep == per-column Thomas in the reference's order
+    (mgMK/mg_relax.F90:484-505, Thomas :541-595)."""
+    levs, _, _ = _levels((6, 4, 10), beta=2.0, factors=[])
+    lev = levs[0]
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((10, 4, 6))
+    f = rng.standard_normal((10, 4, 6))
+    wxm, wxp, dx = lev["cx"][:3]
+    wym, wyp, dy = lev["cy"][:3]
+    wzm, wzp, dz = lev["cz"][:3]
+    ref = u.copy()
+    for j in range(4):
+        for i in range(j % 2, 6, 2):
+            a = wzm.copy(); b = dz + dx[i] + dy[j]; c = wzp.copy()
+            g = (f[:, j, i] - (wxm[i] * u[:, j, (i - 1) % 6] + wxp[i] * u[:, j, (i + 1) % 6]
+                               + wym[j] * u[:, (j - 1) % 4, i] + wyp[j] * u[:, (j + 1) % 4, i]))
+            # the reference's Thomas: sweep down then back up
+            a = a.copy(); b = b.copy(); g = g.copy()
+            for q in range(9):
+                a[q + 1] = -a[q + 1] / b[q]
+                b[q + 1] = b[q + 1] + a[q + 1] * c[q]
+                g[q + 1] = g[q + 1] + a[q + 1] * g[q]
+            g[9] = g[9] / b[9]
+            for q in range(8, -1, -1):
+                g[q] = (g[q] - c[q] * g[q + 1]) / b[q]
+            ref[:, j, i] = g
+    got = VC.zline_half_sweep(u.copy(), f, lev, 0)
+    assert np.abs(got - ref).max() < 1e-12
+
+
+# ---------------------------------------------------------------- transfers
+def test_restriction_invariants():
+    levs, fac, _ = _levels((8, 8, 8), beta=2.0)
+    F, C = levs[0], levs[1]
+    const = np.full((8, 8, 8), 3.5)
+    assert np.abs(VC.restrict(const, F, C, fac[0]) - 3.5).max() < 1e-14
+    rng = np.random.default_rng(8)
+    r = rng.standard_normal((8, 8, 8))
+    rc = VC.restrict(r, F, C, fac[0])
+    assert abs((ops.volumes(C) * rc).sum() - (ops.volumes(F) * r).sum()) < 1e-12
+    # index map: a delta at fine (k,j,i) = (5,2,3) lands on coarse (2,1,1)
+    d = np.zeros((8, 8, 8)); d[5, 2, 3] = 1.0
+    dc = VC.restrict(d, F, C, fac[0])
+    assert np.count_nonzero(dc) == 1 and dc[2, 1, 1] > 0
+    VF = ops.volumes(F); VCc = ops.volumes(C)
+    assert abs(dc[2, 1, 1] - VF[5, 2, 3] / VCc[2, 1, 1]) < 1e-15
+    # uniform grid: exactly the reference's 1/8 average (mgMK/mg_restr.F90:158-161)
+    lu, fu, _ = _levels((8, 8, 8))
+    ru = VC.restrict(r, lu[0], lu[1], fu[0])
+    eighth = r.reshape(4, 2, 4, 2, 4, 2).sum(axis=(1, 3, 5)) / 8.0
+    assert np.abs(ru - eighth).max() < 1e-15
+
+
+def test_prolongation_weights_and_exactness():
+    levs, fac, _ = _levels((8, 8, 8), tbc=(0, 1, 0, 1, 0, 0))
+    F, C = levs[0], levs[1]
+    # constants are reproduced (periodic and Neumann ghosts)
+    e = VC.prolong_add(np.zeros((8, 8, 8)), np.full((4, 4, 4), 2.0), F, C, fac[0])
+    assert np.abs(e - 2.0).max() < 1e-15
+    # a coarse delta spreads 27,9,3,1 /64 (mgMK/mg_cor.F90:102-159)
+    d = np.zeros((4, 4, 4)); d[2, 1, 1] = 64.0
+    e = VC.prolong_add(np.zeros((8, 8, 8)), d, F, C, fac[0])
+    assert e[4, 2, 2] == 27.0 and e[4, 2, 1] == 9.0 and e[3, 2, 1] == 3.0 and e[3, 1, 1] == 1.0
+    assert e.sum() == 64.0 * 8
+    # linear in z on a uniform Neumann z grid: reproduced away from the walls
+    k = np.arange(4)[:, None, None] * np.ones((4, 4, 4))
+    e = VC.prolong_add(np.zeros((8, 8, 8)), k, F, C, fac[0])
+    kf = (np.arange(8) - 0.5) / 2.0
+    assert np.abs(e[1:7] - kf[1:7, None, None]).max() < 1e-15
+    # Dirichlet walls: the coarse ghost is zero
+    ld, fd, _ = _levels((8, 8, 8), tbc=(0, 2, 0, 2, 0, 0))
+    e = VC.prolong_add(np.zeros((8, 8, 8)), np.ones((4, 4, 4)), ld[0], ld[1], fd[0])
+    assert np.abs(e[0] - 0.75).max() < 1e-15 and np.abs(e[3] - 1.0).max() < 1e-15
+
+
+def test_coarse_inverse():
+    for tbc, sing in (((0, 1, 0, 1, 0, 0), True), ((2, 1, 1, 2, 0, 0), False)):
+        levs, _, _ = _levels((4, 4, 4), tbc=tbc, factors=[])
+        lev = levs[0]
+        M = VC.coarse_operator_inverse(lev, sing)
+        rng = np.random.default_rng(9)
+        b = rng.standard_normal(64)
+        if sing:
+            b = ops.compat_project(b.reshape(4, 4, 4), lev).ravel()
+        x = M @ b
+        r = ops.apply_A(x.reshape(4, 4, 4), lev).ravel() - b
+        assert np.abs(r).max() < 1e-11 * np.abs(b).max()
+        if sing:
+            assert abs((ops.volumes(lev).ravel() * x).sum()) < 1e-12
+
+
+# ---------------------------------------------------------------- V-cycle
+@pytest.mark.parametrize("smoother,mixed,maxcy", [("LJ", False, 40), ("LJ", True, 40),
+                                                  ("RBPGS", False, 400)])
+def test_vcycle_converges_to_direct(smoother, mixed, maxcy):
+    levs, fac, _ = _levels((16, 16, 16), beta=2.0)
+    rng = np.random.default_rng(10)
+    b = ops.compat_project(rng.standard_normal((16, 16, 16)), levs[0])
+    x = direct.solve_fft(levs[0], b, True)
+    tbc = [0, 1, 0, 1, 0, 0]
+    p, info = VC.solve(levs, fac, tbc, [-1, 0, -1, 0, -1, -1], np.zeros_like(b), b,
+                       smoother=smoother, mixed=mixed, maxcy=maxcy)
+    assert info["converged"]
+    assert np.abs(p - x).max() <= 1e-9 * np.abs(x).max()
+    if smoother == "LJ":
+        assert info["iters"] <= 12
