@@ -74,8 +74,13 @@ void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStr
 // ---- V-cycle kernels (amg_kernels.hip); fp64 selects the level precision --
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                      int zero_guess, const int* active, hipStream_t s);
+// z-line sweeps run 64 columns per workgroup and 'segs' = ceil(nz_max/kZSeg)
+// waves (segments of kZSeg cells) per column group; every item of one
+// launch must have nz <= segs*kZSeg.
+constexpr int kZSeg = 16;
+constexpr int kZMaxSegs = 16;
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
-                  int zero_guess, const int* active, hipStream_t s);
+                  int zero_guess, int segs, hipStream_t s);
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                      const int* active, hipStream_t s);
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,

@@ -20,6 +20,19 @@ __device__ __forceinline__ int find_item(const int* prefix, int n, int b) {
   return lo;
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);
+// remap so that every XCD walks a contiguous range of tiles and neighbouring
+// rows share an L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_swizzle(int b, int n) {
+  const int q = n >> 3, rem = n & 7, x = b & 7, idx = b >> 3;
+  return x * q + min(x, rem) + idx;
+}
+
+// Reciprocal for the line solves: the hardware rcp (1 ulp) in FP32, a
+// correctly rounded division in FP64.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+
 template <class T> __device__ __forceinline__ const T* coefs(const DevLevel& L);
 template <> __device__ __forceinline__ const double* coefs<double>(const DevLevel& L) { return L.co64; }
 template <> __device__ __forceinline__ const float* coefs<float>(const DevLevel& L) { return L.co32; }
@@ -132,10 +145,11 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                      int parity, int zero_guess,
                                                      const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   const int hx = L.hx, ny = L.ny;
   if (e >= hx * ny * L.nz) return;
   const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
@@ -147,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
   const Nb n = neighbours(L, i, j, k);
   const T d = diag_at<T>(co, L, i, j, k);
   T val;
-  if (zero_guess) {
+  if (zero_guess & 1) {
     val = f[n.c] / d;
   } else {
     const T s = offsum<T>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
@@ -157,23 +171,36 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-538 with
-// Thomas :541-595): thread per z-column of colour (i+j)&1 == parity; the
-// tridiagonal system wzm u_{k-1} + (dx+dy+dz) u_k + wzp u_{k+1} = f - (x/y
-// neighbour terms) is solved exactly.  Forward elimination keeps the
-// modified RHS in u (overwritten by back substitution) and the modified
-// super-diagonal in the scratch array.  A periodic z wrap is lagged.
-template <class T>
-__global__ __launch_bounds__(64) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                              int parity, int zero_guess,
-                                              const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+// z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-538; the
+// reference solves each line with Thomas, :541-595): on every z-column of
+// colour (i+j)&1 == parity solve exactly
+//     wzm_k u_{k-1} + (dz_k + dx_i + dy_j) u_k + wzp_k u_{k+1} = g_k,
+//     g = f - (x/y neighbour terms, other colour).
+// A periodic z wrap is lagged (old values), as the reference lags ghosts.
+//
+// Partitioned solve: the column is cut into segments of S cells; wave p of a
+// workgroup owns segment p of 64 columns (one column per lane, the segment's
+// RHS in registers).  Each segment T_p is solved locally for its interface
+// data -- the first/last entries of y = T_p^-1 g and of the two spikes
+// v = T_p^-1 (a e_1), w = T_p^-1 (c e_S), from one LU (top-down) and one UL
+// (bottom-up) sweep without storage -- then wave 0 solves the 2x2-block
+// reduced system of segment end values in LDS, and every segment re-solves
+// with its end couplings moved to the right-hand side.  All loads are
+// contiguous half-rows of the chequer layout; nothing but u is written.
+template <class T, int S>
+__global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                int parity, int zero_guess) {
+  extern __shared__ unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int col = (blockIdx.x - ll.prefix[t]) * 64 + threadIdx.x;
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
   const int ncol = hx * ny;
-  if (col >= ncol) return;
+  const int lane = threadIdx.x, p = threadIdx.y;
+  const int col0 = (b - ll.prefix[t]) * 64 + lane;
+  const bool live = col0 < ncol;
+  const int col = live ? col0 : ncol - 1;
   const int m = col % hx, j = col / hx;
   const int c = parity, oth = parity ^ 1;
   const int i = 2 * m + ((c + j) & 1), r = i & 1;
@@ -187,46 +214,162 @@ __global__ __launch_bounds__(64) void k_zline(const DevLevel* __restrict__ Lt, L
   const T dxy = co[2 * nx + i] + cy[2 * ny + j];
   T* __restrict__ u = static_cast<T*>(L.u);
   const T* __restrict__ f = static_cast<const T*>(L.f);
-  T* __restrict__ cp = static_cast<T*>(L.scr);
   const size_t plane = (size_t)ny * 2 * hx;
   const size_t own = cidx(m, j, 0, c, hx, ny);
   const size_t xo = cidx(0, j, 0, oth, hx, ny);
   const size_t so = cidx(m, jm, 0, oth, hx, ny);
   const size_t no = cidx(m, jp, 0, oth, hx, ny);
   const bool pz = L.pz != 0;
-  T lagb = T(0), lagt = T(0);
-  if (pz && !zero_guess) {
-    lagb = cz[0] * u[(size_t)(nz - 1) * plane + own];
-    lagt = cz[nz + nz - 1] * u[own];
+  const int Pi = (nz + S - 1) / S;            // segments of this item (<= blockDim.y)
+  const int ks = p * S;
+  const int len = max(0, min(S, nz - ks));    // 0 for spare waves of a shorter item
+  // Rows of the segment, clamped so that every load below is in bounds and
+  // the unrolled loops stay branch-free (all loads in flight at once).
+  // A: sub-diagonal, E: super-diagonal, D: diagonal; the lagged periodic
+  // wrap couplings are zero in the matrix.
+  T A[S], E[S], D[S], g[S];
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const int k = min(ks + q, nz - 1);
+    A[q] = (pz && k == 0) ? T(0) : cz[k];
+    E[q] = (pz && k == nz - 1) ? T(0) : cz[nz + k];
+    D[q] = cz[2 * nz + k] + dxy;
   }
-  T cprev = T(0), dprev = T(0);
-  for (int k = 0; k < nz; ++k) {
+  // zero_guess bit 0: the other colour is zero (first half-sweep from u = 0);
+  // bit 1: this colour's old values are zero (no lagged wrap coupling)
+  const bool nbr = !(zero_guess & 1);
+  const bool lag = pz && !(zero_guess & 2);
+  const T lagb = (lag && p == 0) ? cz[0] * u[(size_t)(nz - 1) * plane + own] : T(0);
+  const T lagt = (lag && ks + len == nz && len > 0) ? cz[2 * nz - 1] * u[own] : T(0);
+#pragma unroll
+  for (int q = 0; q < S; ++q) {
+    const int k = min(ks + q, nz - 1);
     const size_t kb = (size_t)k * plane;
-    T g = f[kb + own];
-    if (!zero_guess) {
+    T v = f[kb + own];
+    if (nbr) {
       T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
       s += wys * u[kb + so];
       s += wyn * u[kb + no];
-      g = g - s;
+      v = v - s;
     }
-    T a = cz[k], cc = cz[nz + k];
-    if (pz) {
-      if (k == 0) { g -= lagb; a = T(0); }
-      if (k == nz - 1) { g -= lagt; cc = T(0); }
-    }
-    const T den = (cz[2 * nz + k] + dxy) - a * cprev;
-    const T cpk = cc / den;
-    const T dpk = (g - a * dprev) / den;
-    cp[(size_t)k * ncol + col] = cpk;
-    u[kb + own] = dpk;
-    cprev = cpk;
-    dprev = dpk;
+    v -= (ks + q == 0) ? lagb : T(0);
+    v -= (ks + q == nz - 1) ? lagt : T(0);
+    g[q] = (q < len) ? v : T(0);
   }
-  T x = dprev;
-  for (int k = nz - 2; k >= 0; --k) {
-    const size_t kb = (size_t)k * plane;
-    x = u[kb + own] - cp[(size_t)k * ncol + col] * x;
-    u[kb + own] = x;
+  const T a_first = A[0];                                  // coupling to u_{ks-1}
+  T c_last = T(0);                                         // coupling to u_{ks+len}
+#pragma unroll
+  for (int q = 0; q < S; ++q) c_last = (q == len - 1) ? E[q] : c_last;
+  // phase 1: interface data of segment p
+  T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
+  if (len > 0) {
+    // LU sweep: yl, vl, wl, w0
+    T cp = T(0), yf = T(0), vf = T(0), prod = T(1);
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      if (q < len) {
+        const T a = (q == 0) ? T(0) : A[q];
+        const T rd = rcp(D[q] - a * cp);
+        yf = (g[q] - a * yf) * rd;
+        vf = (q == 0) ? a_first * rd : -a * vf * rd;
+        if (q == len - 1) {
+          wl = c_last * rd;
+        } else {
+          cp = E[q] * rd;
+          prod *= -cp;
+        }
+      }
+    }
+    yl = yf;
+    vl = vf;
+    w0 = wl * prod;
+    // UL sweep: y0, v0
+    T ri = T(0), h = T(0);
+#pragma unroll
+    for (int q = S - 1; q >= 0; --q) {
+      if (q < len) {
+        if (q < len - 1) {
+          const T ri2 = rcp(D[q] - E[q] * A[q + 1] * ri);
+          h = (g[q] - E[q] * h) * ri2;
+          ri = ri2;
+        } else {
+          ri = rcp(D[q]);
+          h = g[q] * ri;
+        }
+      }
+    }
+    y0 = h;
+    v0 = a_first * ri;
+  }
+  // phase 2: reduced system X_p = (t_p, b_p) (first/last values of segment p)
+  //   t_p + v0_p b_{p-1} + w0_p t_{p+1} = y0_p
+  //   b_p + vl_p b_{p-1} + wl_p t_{p+1} = yl_p
+  const int P = blockDim.y;
+  T* Y0 = lds;
+  T* YL = Y0 + P * 64;
+  T* V0 = YL + P * 64;
+  T* VL = V0 + P * 64;
+  T* W0 = VL + P * 64;
+  T* WL = W0 + P * 64;
+  const int sl = p * 64 + lane;
+  T bprev = T(0), tnext = T(0);
+  if (Pi > 1) {
+    Y0[sl] = y0; YL[sl] = yl; V0[sl] = v0; VL[sl] = vl; W0[sl] = w0; WL[sl] = wl;
+    __syncthreads();
+    if (p == 0) {
+      // downward: b_{q-1} = beta - gamma t_q ;  t_q = tau - sigma t_{q+1}
+      T beta = T(0), gamma = T(0);
+      for (int q = 0; q < Pi; ++q) {
+        const int s2 = q * 64 + lane;
+        const T dn = T(1) / (T(1) - V0[s2] * gamma);
+        const T tau = (Y0[s2] - V0[s2] * beta) * dn;
+        const T sigma = W0[s2] * dn;
+        const T nb = YL[s2] - VL[s2] * beta + VL[s2] * gamma * tau;
+        const T ng = WL[s2] + VL[s2] * gamma * sigma;
+        V0[s2] = tau; W0[s2] = sigma; VL[s2] = nb; WL[s2] = ng;
+        beta = nb; gamma = ng;
+      }
+      // upward: t_P = 0
+      T tn = T(0);
+      for (int q = Pi - 1; q >= 0; --q) {
+        const int s2 = q * 64 + lane;
+        const T tq = V0[s2] - W0[s2] * tn;
+        const T bq = VL[s2] - WL[s2] * tn;
+        Y0[s2] = tq;
+        YL[s2] = bq;
+        tn = tq;
+      }
+    }
+    __syncthreads();
+    bprev = (p > 0) ? YL[(p - 1) * 64 + lane] : T(0);
+    tnext = (p < Pi - 1) ? Y0[(p + 1) * 64 + lane] : T(0);
+  }
+  // phase 3: re-solve the segment with its end couplings on the right-hand side
+  if (len > 0) {
+    T cpa[S];
+    T cp = T(0), yf = T(0);
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      if (q < len) {
+        T rhs = g[q];
+        if (q == 0) rhs -= a_first * bprev;
+        if (q == len - 1) rhs -= c_last * tnext;
+        const T a = (q == 0) ? T(0) : A[q];
+        const T rd = rcp(D[q] - a * cp);
+        yf = (rhs - a * yf) * rd;
+        cp = (q == len - 1) ? T(0) : E[q] * rd;
+        g[q] = yf;
+        cpa[q] = cp;
+      }
+    }
+    T x = T(0);
+#pragma unroll
+    for (int q = S - 1; q >= 0; --q) {
+      if (q < len) {
+        x = g[q] - cpa[q] * x;
+        if (live) u[(size_t)(ks + q) * plane + own] = x;
+      }
+    }
   }
 }
 
@@ -238,12 +381,13 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict__ Lt,
                                                      const DevXfer* __restrict__ X, LaunchList ll,
                                                      const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevXfer x = X[ll.items[t]];
   const DevLevel& F = Lt[x.fine];
   const DevLevel& C = Lt[x.coarse];
   if (active && !active[F.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= C.ncell) return;
   const int I = e % C.nx, JK = e / C.nx, J = JK % C.ny, K = JK / C.ny;
   const T* __restrict__ co = coefs<T>(F);
@@ -297,12 +441,13 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__ Lt,
                                                     const DevXfer* __restrict__ X, LaunchList ll,
                                                     const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevXfer x = X[ll.items[t]];
   const DevLevel& F = Lt[x.fine];
   const DevLevel& C = Lt[x.coarse];
   if (active && !active[F.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= F.ncell) return;
   int i, j, k;
   decode(F, e, i, j, k);
@@ -336,10 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                    const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int r = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int r = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   const int n = L.ncell;
   if (r >= n) return;
   const T* __restrict__ f = static_cast<const T*>(L.f);
@@ -354,10 +500,11 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
 __global__ __launch_bounds__(kBlock) void k_residual_max(const DevLevel* __restrict__ Lt,
                                                          LaunchList ll, double* __restrict__ partial,
                                                          const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (e < L.ncell) {
     int i, j, k;
@@ -370,16 +517,17 @@ __global__ __launch_bounds__(kBlock) void k_residual_max(const DevLevel* __restr
     v[0] = fabs(f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]));
     v[1] = fabs(f[n.c]);
   }
-  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
 }
 
 __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restrict__ Lt,
                                                        LaunchList ll, double* __restrict__ partial,
                                                        const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (e < L.ncell) {
     const double a = static_cast<const double*>(L.u)[e];
@@ -387,16 +535,17 @@ __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restric
     v[0] = fabs(a - b);
     v[1] = fabs(a);
   }
-  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
 }
 
 __global__ __launch_bounds__(kBlock) void k_defect32(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                      double* __restrict__ partial,
                                                      const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (e < L.ncell) {
     int i, j, k;
@@ -411,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void k_defect32(const DevLevel* __restrict_
     v[0] = fabs(r);
     v[1] = fabs(f[n.c]);
   }
-  block_reduce_store<2, false>(v, partial + (size_t)blockIdx.x * 2);
+  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
 }
 
 // p_out = p_in + e, then r = b - A p_out from p_in and e at the seven points
@@ -420,10 +569,11 @@ __global__ __launch_bounds__(kBlock) void k_correct_defect32(const DevLevel* __r
                                                              LaunchList ll,
                                                              double* __restrict__ partial,
                                                              const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (e < L.ncell) {
     int i, j, k;
@@ -445,21 +595,34 @@ __global__ __launch_bounds__(kBlock) void k_correct_defect32(const DevLevel* __r
     v[1] = fabs((double)ev[n.c]);
     v[2] = fabs(pc);
   }
-  block_reduce_store<3, false>(v, partial + (size_t)blockIdx.x * 3);
+  block_reduce_store<3, false>(v, partial + (size_t)b * 3);
 }
 
 // Per-item reduction of block partials (fixed order: deterministic).
-__global__ void k_finalize(LaunchList ll, const double* __restrict__ partial, int nvals, int op_sum,
-                           double* __restrict__ out, int ostride) {
+// One block per item: thread t folds partials b0+t, b0+t+256, ... in order,
+// then a fixed-shape tree combines the threads (deterministic).
+__global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
+                                                     int nvals, int op_sum, double* __restrict__ out,
+                                                     int ostride) {
+  __shared__ double sm[kBlock / 64];
   const int t = blockIdx.x;
   const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
-  for (int q = threadIdx.x; q < nvals; q += blockDim.x) {
-    double a = op_sum ? 0.0 : 0.0;
-    for (int b = b0; b < b1; ++b) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = 0; q < nvals; ++q) {
+    double a = 0.0;
+    for (int b = b0 + threadIdx.x; b < b1; b += kBlock) {
       const double x = partial[(size_t)b * nvals + q];
       a = op_sum ? a + x : nmax(a, x);
     }
-    out[(size_t)t * ostride + q] = a;
+    a = op_sum ? wave_sum(a) : wave_max(a);
+    if (lane == 0) sm[w] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double r = sm[0];
+      for (int i = 1; i < kBlock / 64; ++i) r = op_sum ? r + sm[i] : nmax(r, sm[i]);
+      out[(size_t)t * ostride + q] = r;
+    }
+    __syncthreads();
   }
 }
 
@@ -472,9 +635,10 @@ __device__ __forceinline__ size_t fidx(int i, int j, int k, int nx, int ny) {
 __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restrict__ Lt,
                                                          LaunchList ll, PtrList ptr,
                                                          int which, int add_fold) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
@@ -488,9 +652,10 @@ __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restr
 
 __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                  int which, double* __restrict__ partial) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   double v[1] = {0.0};
   if (e < L.ncell) {
     int i, j, k;
@@ -500,15 +665,16 @@ __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt
                                   : static_cast<const double*>(which == 0 ? L.u : L.f)[e];
     v[0] = w * x;
   }
-  block_reduce_store<1, true>(v, partial + blockIdx.x);
+  block_reduce_store<1, true>(v, partial + b);
 }
 
 __global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                   int which, const double* __restrict__ c,
                                                   int with_rho) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
   double* x = static_cast<double*>(which == 0 ? L.u : L.f);
   double s = c[t] / (with_rho ? L.vrsum : L.vsum);
@@ -522,9 +688,10 @@ __global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ L
 
 __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restrict__ Lt,
                                                        LaunchList ll, PtrList ptr) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
@@ -533,19 +700,21 @@ __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restric
 
 __global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                    const int* __restrict__ active) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (active && !active[L.inst]) return;
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
   static_cast<double*>(L.u2)[e] = static_cast<const double*>(L.u)[e];
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_fill_zero(const DevLevel* __restrict__ Lt, LaunchList ll) {
-  const int t = find_item(ll.prefix, ll.nitems, blockIdx.x);
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (blockIdx.x - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.ncell) static_cast<T*>(L.u)[e] = T(0);
 }
 
@@ -721,9 +890,13 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
   else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess, active);
 }
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
-                  const int* active, hipStream_t s) {
-  if (fp64) LAUNCH(k_zline<double>, ll.nblocks, 64, L, ll, parity, zero_guess, active);
-  else LAUNCH(k_zline<float>, ll.nblocks, 64, L, ll, parity, zero_guess, active);
+                  int segs, hipStream_t s) {
+  const dim3 blk(64, segs);
+  const size_t lds = (size_t)6 * segs * 64 * (fp64 ? 8 : 4);
+  if (fp64)
+    hipLaunchKernelGGL((k_zline<double, kZSeg>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess);
+  else
+    hipLaunchKernelGGL((k_zline<float, kZSeg>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess);
 }
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                      const int* active, hipStream_t s) {
@@ -758,7 +931,7 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 }
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
                      int ostride, hipStream_t s) {
-  LAUNCH(k_finalize, ll.nitems, 64, ll, partial, nvals, op_sum, out, ostride);
+  LAUNCH(k_finalize, ll.nitems, kBlock, ll, partial, nvals, op_sum, out, ostride);
 }
 void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, int which,
                          int add_fold, hipStream_t s) {

@@ -18,11 +18,6 @@ namespace amg {
 namespace {
 thread_local std::string g_last_error;
 
-struct Item {
-  int idx;
-  int blocks;
-};
-
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // d_out/h_out hold four regions of kOutRegion doubles: [0] per-instance norms
@@ -128,6 +123,11 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     delete I;
     return rc;
   }
+  if (cfg.smoother == AMG_SMOOTHER_LJ && g.nz > kZSeg * kZMaxSegs) {
+    set_error("amg_create: the z-line smoother supports nz <= 256");
+    delete I;
+    return AMG_EBADSIZE;
+  }
   I->id = inst.empty() ? 0 : inst.back()->id + 1;
   I->nx = g.nx; I->ny = g.ny; I->nz = g.nz;
   I->fp64 = (cfg.precision == AMG_FP64);
@@ -184,8 +184,8 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     // fields of the V-cycle precision
     d.u = I->mem.alloc((size_t)d.ncell * w);
     d.f = I->mem.alloc((size_t)d.ncell * w);
-    d.scr = I->mem.alloc((size_t)(d.ncell / 2 + 1) * w);
-    if (!d.u || !d.f || !d.scr) return fail();
+    d.scr = nullptr;
+    if (!d.u || !d.f) return fail();
   }
   // FP64 finest entries
   DevLevel fA = ents[0], fB = ents[0];
@@ -275,6 +275,21 @@ int Context::upload_tables() {
   return AMG_OK;
 }
 
+// z-line launches: items grouped by their segment count (blockDim.y).
+void Context::zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg) {
+  std::vector<Item> bucket[kZMaxSegs + 1];
+  for (const Item& it : items) {
+    const int nz = hL[it.idx].nz;
+    const int segs = std::min(kZMaxSegs, (nz + kZSeg - 1) / kZSeg);
+    bucket[segs].push_back(it);
+  }
+  for (int sg = 1; sg <= kZMaxSegs; ++sg)
+    if (!bucket[sg].empty())
+      for_lists(bucket[sg], [&](const LaunchList& ll, int, int) {
+        launch_zline(fp64, dL, ll, parity, zg, sg, stream);
+      });
+}
+
 // ---------------------------------------------------------------------------
 void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   if (S.empty()) return;
@@ -283,9 +298,9 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   for (Instance* I : S) maxL = std::max(maxL, (int)I->lev.size());
   const DevLevel* L = dL;
   auto half = [&](const std::vector<Item>& items, int parity, int zg) {
-    for_lists(items, [&](const LaunchList& ll, int, int) {
-      if (zline) launch_zline(fp64, L, ll, parity, zg, nullptr, stream);
-      else launch_rb_point(fp64, L, ll, parity, zg, nullptr, stream);
+    if (zline) zline_launches(items, fp64, parity, zg);
+    else for_lists(items, [&](const LaunchList& ll, int, int) {
+      launch_rb_point(fp64, L, ll, parity, zg, nullptr, stream);
     });
   };
   auto blocks_smooth = [&](const DevLevel& d) {
@@ -303,8 +318,10 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
         const int li = I->lev[dpt];
         items.push_back({li, blocks_smooth(hL[li])});
       }
-      half(items, 0, (zero_first && s == 0) ? 1 : 0);
-      half(items, 1, 0);
+      // from a zero initial guess: the red pass sees zero everywhere, the
+      // black pass sees zero old black values (lagged periodic-z couplings)
+      half(items, 0, (zero_first && s == 0) ? 3 : 0);
+      half(items, 1, (zero_first && s == 0) ? 2 : 0);
     }
   };
   for (int dpt = 0; dpt + 1 < maxL; ++dpt) {
@@ -647,8 +664,8 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
   for (int r = 0; r < reps; ++r) {
     for_lists(items, [&](const LaunchList& ll, int, int) {
       if (which == 0 || which == 4 || which == 6) {
-        const int par = which == 4 ? 1 : 0, zg = which == 6 ? 1 : 0;
-        if (zline) launch_zline(fp64, dL, ll, par, zg, nullptr, stream);
+        const int par = which == 4 ? 1 : 0, zg = which == 6 ? 3 : 0;
+        if (zline) zline_launches(items, fp64, par, zg);
         else launch_rb_point(fp64, dL, ll, par, zg, nullptr, stream);
       } else if (which == 1) {
         launch_restrict(fp64, dL, dX, ll, nullptr, stream);

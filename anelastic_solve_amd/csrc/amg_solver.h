@@ -27,6 +27,12 @@ constexpr int kMaxInstances = 1024;
     }                                                                        \
   } while (0)
 
+// One work-list entry: descriptor index and its block count.
+struct Item {
+  int idx;
+  int blocks;
+};
+
 // Owns device allocations.
 struct DevArena {
   std::vector<void*> ptrs;
@@ -87,6 +93,7 @@ struct Context {
   int solve(const std::vector<Instance*>& S, amg_stats* st);
   // pieces
   void vcycle(const std::vector<Instance*>& S, bool fp64);
+  void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
   int prepare(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
   int cycle_and_norms(const std::vector<Instance*>& S);

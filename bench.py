@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the pressure-Poisson solve on one MI355X.
+
+Workload (the headline configuration): 256^3 cells on 2*pi x 2*pi x 1,
+periodic x/y, Neumann z, tanh vertical stretching (beta = 2, clustered at
+the mid-plane), RHS = divergence of a seeded random velocity with a k^-5/3
+spectrum in x/y.  A step is one solve from a zero initial guess to a
+relative residual ||b - A p||_inf / ||b||_inf <= 1e-10 with V(2,2) cycles
+and the z-line red-black Gauss-Seidel smoother, inputs resident in HBM.
+
+``value`` is solves/s of the FP32-smoothed / FP64-corrected solver; the
+plain FP64 V-cycle runs on the same input for the comparison.  The JSON
+line also carries the dominant kernel's roofline (HIP events on the
+solver's stream around back-to-back launches), V-cycles/s, algorithmic
+GB/s of the whole cycle, per-solve p50/p99 latency, the pressure error of
+the mixed solve against a tight FP64 solve, and a CPU baseline (the numpy
+oracle's V-cycle timed on the host on a bounded sample).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("solves/s + V-cycle HBM GB/s at 256³ to 1e-10, mixed vs FP64; p50/p99 latency")
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, nargs=3, default=[256, 256, 256])
+    ap.add_argument("--beta", type=float, default=2.0)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--smoother", default="LJ")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison")
+    ap.add_argument("--cpu-cycles", type=int, default=1)
+    return ap.parse_args()
+
+
+def percentile(xs, q):
+    ys = sorted(xs)
+    if not ys:
+        return float("nan")
+    k = (len(ys) - 1) * q / 100.0
+    lo, hi = int(math.floor(k)), int(math.ceil(k))
+    return ys[lo] + (ys[hi] - ys[lo]) * (k - lo)
+
+
+def run_solves(torch, solver, rhs, steps, warmup):
+    """Warmup then time exactly `steps` solves; returns (times_s, last_p, stats)."""
+    p = torch.zeros_like(rhs)
+    for _ in range(warmup):
+        p.zero_()
+        solver.mg_solver(p, rhs)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.zero_()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        solver.mg_solver(p, rhs)
+        times.append(time.perf_counter() - ts)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    return times, total, p
+
+
+def kernel_roofline(torch, solver, reps):
+    """Average duration of the dominant kernel (finest-level smoother half-sweep)
+    from HIP events recorded on the solver's own stream."""
+    stream = torch.cuda.ExternalStream(solver.stream)
+    solver.run_kernel(0, 0, 3)            # warm the kernel
+    stream.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    nbytes = solver.run_kernel(0, 0, reps)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "kernel": "k_zline<float> finest level, one colour (red half-sweep)",
+            "bytes_per_launch": nbytes, "avg_launch_us": ms * 1e3}
+
+
+def cpu_baseline(n, beta, tol, cycles, iters_needed):
+    """Bounded sample: `cycles` V-cycles of the numpy oracle (mixed precision,
+    same grid and RHS) timed on the host; solves/s extrapolated with the GPU's
+    cycle count."""
+    import numpy as np
+    from anelastic_solve_amd import box_grid, workloads
+    from oracle import grid as G, ops, vcycle as VC
+    g = box_grid(*n, beta=beta)
+    U, V, W = workloads.velocity_spectral(g, 1)
+    b = workloads.psource_host(U, V, W, g)[1:-1, 1:-1, 1:-1]
+    ax = G.axis_from_faces(g.ax.xe[:g.nx + 1])
+    ay = G.axis_from_faces(g.ay.xe[:g.ny + 1])
+    az = G.axis_from_faces(g.az.xe[:g.nz + 1])
+    fac = G.auto_factors(*n)
+    tbc = [0, 1, 0, 1, 0, 0]
+    levs = G.hierarchy(ax, ay, az, np.ones(n[2] + 2), tbc, fac)
+    bb = ops.compat_project(b, levs[0])
+    Minv = VC.coarse_operator_inverse(levs[-1], True)
+    L = len(levs)
+    p = np.zeros_like(bb)
+    t0 = time.perf_counter()
+    for _ in range(cycles):
+        r = VC.residual(p, bb, levs[0])
+        e = np.zeros(p.shape, dtype=np.float32)
+        VC.vcycle(levs, fac, e, r.astype(np.float32), [2] * L, [2] * L, VC.LJ, np.float32, Minv, 0,
+                  zero_guess=True)
+        p = p + e
+    dt = (time.perf_counter() - t0) / cycles
+    return {"value": 1.0 / (dt * iters_needed), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"{cycles} mixed V(2,2) z-line cycle(s) of the numpy oracle at "
+                       f"{n[0]}x{n[1]}x{n[2]}; {dt:.2f} s/cycle x {iters_needed} cycles per solve")}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note, not measured)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
+
+    n = tuple(a.n)
+    g = box_grid(*n, beta=a.beta)
+    U, V, W = workloads.velocity_spectral(g, 1)
+    rhs = torch.from_numpy(workloads.psource_host(U, V, W, g)).cuda()
+    print(f"# workload {n} beta={a.beta} ready", file=sys.stderr, flush=True)
+
+    res = {}
+    sol = {}
+    for prec in (["mixed"] if a.no_fp64 else ["mixed", "fp64"]):
+        cfg = default_config(precision=prec, smoother=a.smoother, tol=a.tol, maxcy=200)
+        s = PressureSolver(g, cfg)
+        times, total, p = run_solves(torch, s, rhs, a.steps, a.warmup)
+        st = s.stats
+        r = {"solves_per_s": a.steps / total, "ms_per_solve": 1e3 * total / a.steps,
+             "iters": st.iters, "relres": st.relres, "converged": bool(st.converged),
+             "ms_cycles": st.ms_cycles, "ms_total_device": st.ms_total,
+             "bytes_cycles": st.bytes_cycles,
+             "cycle_GBps": st.bytes_cycles / (st.ms_cycles * 1e-3) / 1e9 if st.ms_cycles > 0 else None,
+             "vcycles_per_s": st.iters / (st.ms_cycles * 1e-3) if st.ms_cycles > 0 else None,
+             "p50_ms": 1e3 * percentile(times, 50), "p99_ms": 1e3 * percentile(times, 99)}
+        if prec == "mixed":
+            r["roofline"] = kernel_roofline(torch, s, a.kernel_reps)
+            r["levels"] = s.levels
+        res[prec] = r
+        sol[prec] = p.clone()
+        print(f"# {prec}: {r['solves_per_s']:.1f} solves/s, {st.iters} cycles, "
+              f"relres {st.relres:.2e}", file=sys.stderr, flush=True)
+        s.close()
+
+    # pressure error of the mixed solve against a tight FP64 solve (null space removed by both)
+    cfg = default_config(precision="fp64", smoother=a.smoother, tol=1e-13, maxcy=200)
+    s = PressureSolver(g, cfg)
+    pt = torch.zeros_like(rhs)
+    s.mg_solver(pt, rhs)
+    tight_relres = s.stats.relres
+    s.close()
+    inner = (slice(1, -1),) * 3
+    err = {k: (sol[k][inner] - pt[inner]).abs().max().item() / pt[inner].abs().max().item()
+           for k in sol}
+
+    m = res["mixed"]
+    line = {
+        "metric": METRIC,
+        "value": m["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": m["ms_per_solve"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "stretched-256: tanh z (beta %.1f), k^-5/3 RHS, periodic x/y, "
+                               "Neumann z" % a.beta,
+                   "grid": list(n), "domain": [2 * math.pi, 2 * math.pi, 1.0],
+                   "smoother": a.smoother, "cycle": "V(2,2)", "tol": a.tol,
+                   "precision": "FP32 V-cycle inside FP64 defect correction",
+                   "levels": m["levels"]},
+        "roofline": m["roofline"],
+        "iters": m["iters"],
+        "relres": m["relres"],
+        "vcycles_per_s": m["vcycles_per_s"],
+        "cycle_achieved_GBps": m["cycle_GBps"],
+        "cycle_roofline_frac": (m["cycle_GBps"] / HBM_PEAK_GBPS) if m["cycle_GBps"] else None,
+        "latency_ms": {"p50": m["p50_ms"], "p99": m["p99_ms"]},
+        "pressure_err_vs_fp64_tight": err,
+        "fp64_tight_relres": tight_relres,
+    }
+    if "fp64" in res:
+        f = res["fp64"]
+        line["fp64"] = {k: f[k] for k in ("solves_per_s", "ms_per_solve", "iters", "relres",
+                                         "vcycles_per_s", "cycle_GBps", "p50_ms", "p99_ms")}
+        line["speedup_mixed_vs_fp64"] = m["solves_per_s"] / f["solves_per_s"]
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, a.beta, a.tol, a.cpu_cycles, m["iters"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

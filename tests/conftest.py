@@ -28,13 +28,16 @@ def cuda():
     return torch
 
 
-def oracle_levels(grid, tbc, solver_levels=None):
+def oracle_levels(grid, tbc, solver_levels=None, nlevels=None):
     """Oracle hierarchy built from the same metrics as the product grid."""
     from oracle import grid as G
     ax = G.axis_from_faces(grid.ax.xe[:grid.nx + 1])
     ay = G.axis_from_faces(grid.ay.xe[:grid.ny + 1])
     az = G.axis_from_faces(grid.az.xe[:grid.nz + 1])
-    fac = G.auto_factors(grid.nx, grid.ny, grid.nz)
+    if nlevels is None:
+        fac = G.auto_factors(grid.nx, grid.ny, grid.nz)
+    else:
+        fac = G.reference_factors(nlevels)
     levs = G.hierarchy(ax, ay, az, grid.rho_or_ones(), list(tbc), fac)
     if solver_levels is not None:
         assert [l["n"] for l in levs] == [tuple(s) for s in solver_levels]

@@ -16,15 +16,20 @@ from conftest import oracle_levels, relerr
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # (n, beta, tbc)
-    ((16, 16, 16), 2.0, (0, 1, 0, 1, 0, 0)),          # periodic x/y, Neumann z, stretched
-    ((16, 8, 12), 0.0, (0, 1, 0, 1, 0, 0)),           # uneven sizes
-    ((8, 8, 8), 1.0, (2, 2, 1, 1, 2, 2)),             # Dirichlet x/y faces, Neumann z
-    ((16, 16, 8), 0.0, (0, 0, 0, 0, 0, 0)),           # fully periodic
+    # (n, beta, tbc, nlevels)
+    ((16, 16, 16), 2.0, (0, 1, 0, 1, 0, 0), None),          # periodic x/y, Neumann z, stretched
+    ((16, 8, 12), 0.0, (0, 1, 0, 1, 0, 0), None),           # uneven sizes
+    ((8, 8, 8), 1.0, (2, 2, 1, 1, 2, 2), None),             # Dirichlet x/y faces, Neumann z
+    ((16, 16, 8), 0.0, (0, 0, 0, 0, 0, 0), None),           # fully periodic
+    # z-columns longer than one 16-cell segment (partitioned line solve)
+    ((8, 8, 64), 2.0, (0, 1, 0, 1, 0, 0), None),            # 4 segments
+    ((16, 16, 36), 1.0, (2, 2, 1, 1, 2, 2), None),          # 3 segments, tail of 4
+    ((8, 8, 48), 0.0, (0, 0, 0, 0, 0, 0), None),            # periodic z across segments
+    ((8, 8, 17), 1.5, (0, 1, 0, 1, 0, 0), (2, 2, 1)),       # tail of 1 cell, z not coarsened
 ]
 
 
-def _solver(n, beta, tbc, smoother, precision, rho=False):
+def _solver(n, beta, tbc, smoother, precision, rho=False, nlevels=None):
     from anelastic_solve_amd import PressureSolver, box_grid, default_config
     nx, ny, nz = n
     r = None
@@ -33,9 +38,10 @@ def _solver(n, beta, tbc, smoother, precision, rho=False):
         r = 1.0 - 0.1 * z
     g = box_grid(nx, ny, nz, beta=beta, rho=r)
     vbc = [0.0 if t else -1.0 for t in tbc]
-    cfg = default_config(smoother=smoother, precision=precision, tbc=tbc, vbc=vbc)
+    kw = {} if nlevels is None else {"nlevels": nlevels}
+    cfg = default_config(smoother=smoother, precision=precision, tbc=tbc, vbc=vbc, **kw)
     s = PressureSolver(g, cfg)
-    levs, fac = oracle_levels(g, tbc, s.levels)
+    levs, fac = oracle_levels(g, tbc, s.levels, nlevels)
     return s, g, levs, fac
 
 
@@ -50,8 +56,8 @@ def _tol(precision, smoother):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_half_sweeps(cuda, case, smoother, precision):
     from oracle import vcycle as VC
-    n, beta, tbc = CASES[case]
-    s, g, levs, fac = _solver(n, beta, tbc, smoother, precision, rho=(case == 0))
+    n, beta, tbc, nlev = CASES[case]
+    s, g, levs, fac = _solver(n, beta, tbc, smoother, precision, rho=(case in (0, 4)), nlevels=nlev)
     dt = np.float64 if precision == "fp64" else np.float32
     rng = np.random.default_rng(100 + case)
     L = len(levs)
@@ -81,8 +87,8 @@ def test_half_sweeps(cuda, case, smoother, precision):
 def test_transfers_and_coarse_solve(cuda, case, precision):
     from oracle import vcycle as VC
     from oracle.grid import is_singular
-    n, beta, tbc = CASES[case]
-    s, g, levs, fac = _solver(n, beta, tbc, "LJ", precision, rho=(case == 0))
+    n, beta, tbc, nlev = CASES[case]
+    s, g, levs, fac = _solver(n, beta, tbc, "LJ", precision, rho=(case in (0, 4)), nlevels=nlev)
     dt = np.float64 if precision == "fp64" else np.float32
     tol = 1e-12 if precision == "fp64" else 2e-5
     rng = np.random.default_rng(200 + case)
