@@ -69,11 +69,32 @@ struct DevXfer {
 
 constexpr int kBlock = 256;
 
+// Coarse tail: the levels of an instance at and below the first level with
+// at most kTailCells cells (and nz <= kTailNz for the z-line smoother) are
+// cycled inside one workgroup of 64*kTailWaves
+// threads (one launch per V-cycle instead of ~10 per level).
+constexpr int kTailCells = 32768;
+constexpr int kTailNz = 32;      // z-line tail levels: whole columns in registers (FP32)
+constexpr int kTailNz64 = 16;    // the same for FP64 (register budget)
+constexpr int kTailWaves = 8;
+constexpr int kTailMaxLevels = 16;
+struct DevTail {
+  int nlev;                       // tail levels, the last is the coarsest
+  int zero_first;                 // the first tail level starts from u = 0
+  int copy_u_in;                  // the first tail level's u is read (not zero)
+  int zline;                      // smoother: 1 z-line, 0 point red-black
+  int lds_u[kTailMaxLevels];      // LDS offsets (elements of the level precision)
+  int lds_f[kTailMaxLevels];
+  int lev[kTailMaxLevels];        // DevLevel index per tail level
+  int xf[kTailMaxLevels];         // DevXfer index of tail level d -> d+1
+  int pre[kTailMaxLevels], post[kTailMaxLevels];
+};
+
 void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
 
 // ---- V-cycle kernels (amg_kernels.hip); fp64 selects the level precision --
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
-                     int zero_guess, const int* active, hipStream_t s);
+                     int zero_guess, hipStream_t s);
 // z-line sweeps run 64 columns per workgroup and 'segs' = ceil(nz_max/kZSeg)
 // waves (segments of kZSeg cells) per column group; every item of one
 // launch must have nz <= segs*kZSeg.
@@ -82,28 +103,38 @@ constexpr int kZMaxSegs = 16;
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                   int zero_guess, int segs, hipStream_t s);
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                     const int* active, hipStream_t s);
+                     hipStream_t s);
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                    const int* active, hipStream_t s);
-void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, const int* active,
-                   hipStream_t s);
+                    hipStream_t s);
+void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
+// the coarse tail's short-column z-line sweep as a standalone launch (tests)
+void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                      int zero_guess, hipStream_t s);
+// one workgroup per item (items index DevTail entries), lds_bytes of
+// dynamic LDS (the largest tail of the items); tail_lds_limit() is the most
+// a launch may request on this device.
+void launch_tail(bool fp64, const DevLevel* L, const DevXfer* X, const DevTail* T,
+                 const LaunchList& ll, size_t lds_bytes, hipStream_t s);
+size_t tail_lds_limit();
 
 // ---- FP64 finest-level kernels (items index FP64 finest entries) -------------
-// Each block writes nvals partial maxima to partial[block*nvals + v];
-// launch_finalize reduces them per item into out[item*ostride + v].
+// Each thread handles kFinePer cells; each block writes nvals partial maxima
+// to partial[block*nvals + v] (blocks per item: cdiv(ncell, kBlock*kFinePer));
+// launch_finalize folds them per item into out[item*ostride + v].
 // residual_max:     {max|b - A u|, max|b|}
 // change_max:       {max|u - u2|, max|u|}        (u2 holds the previous iterate)
 // defect32:         r = b - A u -> FP32 f of entry 'link'; {max|r|, max|b|}
 // correct_defect32: u2 = u + e (e = FP32 u of 'link'); r = b - A u2 -> FP32 f;
 //                   {max|r|, max|e|, max|u2|}
-void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial,
-                         const int* active, hipStream_t s);
-void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial,
-                       const int* active, hipStream_t s);
-void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
-                     const int* active, hipStream_t s);
+constexpr int kFinePer = 8;
+void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
+void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
+void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
-                             const int* active, hipStream_t s);
+                             hipStream_t s);
+void launch_zero(double* a, int n, hipStream_t s);
+// weighted sums fold kWsumPer cells per thread: blocks per item = cdiv(ncell, kBlock*kWsumPer)
+constexpr int kWsumPer = 16;
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum,
                      double* out, int ostride, hipStream_t s);
 

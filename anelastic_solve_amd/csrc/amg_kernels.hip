@@ -138,20 +138,12 @@ __device__ __forceinline__ T diag_at(const T* __restrict__ co, const DevLevel& L
 
 // ---------------------------------------------------------------------------
 // Point red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:146-168):
-// u = (f - sum w_n u_n)/diag on the cells with (i+j+k)&1 == parity.  Thread
-// per cell of the colour; in plane k the colour is column colour (parity+k)&1,
-// so each thread reads one contiguous half-row position of f and u.
+// u = (f - sum w_n u_n)/diag on the cells with (i+j+k)&1 == parity.  Element
+// e enumerates the cells of the colour; in plane k the colour is column
+// colour (parity+k)&1, so consecutive e are one contiguous half-row.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                     int parity, int zero_guess,
-                                                     const int* __restrict__ active) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
+__device__ __forceinline__ void rb_point_elem(const DevLevel& L, int e, int parity, int zero_guess) {
   const int hx = L.hx, ny = L.ny;
-  if (e >= hx * ny * L.nz) return;
   const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
   const int c = (k + parity) & 1;
   const int i = 2 * m + ((c + j) & 1);
@@ -170,6 +162,16 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
   u[n.c] = val;
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                     int parity, int zero_guess) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e < L.hx * L.ny * L.nz) rb_point_elem<T>(L, e, parity, zero_guess);
+}
+
 // ---------------------------------------------------------------------------
 // z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-538; the
 // reference solves each line with Thomas, :541-595): on every z-column of
@@ -179,27 +181,24 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
 // A periodic z wrap is lagged (old values), as the reference lags ghosts.
 //
 // Partitioned solve: the column is cut into segments of S cells; wave p of a
-// workgroup owns segment p of 64 columns (one column per lane, the segment's
-// RHS in registers).  Each segment T_p is solved locally for its interface
-// data -- the first/last entries of y = T_p^-1 g and of the two spikes
-// v = T_p^-1 (a e_1), w = T_p^-1 (c e_S), from one LU (top-down) and one UL
-// (bottom-up) sweep without storage -- then wave 0 solves the 2x2-block
-// reduced system of segment end values in LDS, and every segment re-solves
-// with its end couplings moved to the right-hand side.  All loads are
-// contiguous half-rows of the chequer layout; nothing but u is written.
+// column group owns segment p of 64 columns (one column per lane, the
+// segment's RHS in registers).  Each segment T_p is solved locally for its
+// interface data -- the first/last entries of y = T_p^-1 g and of the two
+// spikes v = T_p^-1 (a e_1), w = T_p^-1 (c e_S), from one LU (top-down) and
+// one UL (bottom-up) sweep without storage -- then wave 0 solves the
+// 2x2-block reduced system of segment end values in LDS, and every segment
+// re-solves with its end couplings moved to the right-hand side.  All loads
+// are contiguous half-rows of the chequer layout; nothing but u is written.
+// One column group: 64 columns (lane = column) x P segments (wave p).  Every
+// wave of the workgroup must call it together: it synchronises the
+// workgroup when a column has more than one segment.  'lds' holds 6*P*64
+// values of T for this group; 'act' false makes the group idle.
 template <class T, int S>
-__global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                int parity, int zero_guess) {
-  extern __shared__ unsigned char lds_raw[];
-  T* lds = reinterpret_cast<T*>(lds_raw);
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevLevel& L = Lt[ll.items[t]];
+__device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int zero_guess, int col0,
+                                            int p, int lane, int P, T* __restrict__ lds, bool act) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
   const int ncol = hx * ny;
-  const int lane = threadIdx.x, p = threadIdx.y;
-  const int col0 = (b - ll.prefix[t]) * 64 + lane;
-  const bool live = col0 < ncol;
+  const bool live = act && col0 < ncol;
   const int col = live ? col0 : ncol - 1;
   const int m = col % hx, j = col / hx;
   const int c = parity, oth = parity ^ 1;
@@ -304,7 +303,6 @@ __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt,
   // phase 2: reduced system X_p = (t_p, b_p) (first/last values of segment p)
   //   t_p + v0_p b_{p-1} + w0_p t_{p+1} = y0_p
   //   b_p + vl_p b_{p-1} + wl_p t_{p+1} = yl_p
-  const int P = blockDim.y;
   T* Y0 = lds;
   T* YL = Y0 + P * 64;
   T* V0 = YL + P * 64;
@@ -343,6 +341,7 @@ __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt,
     __syncthreads();
     bprev = (p > 0) ? YL[(p - 1) * 64 + lane] : T(0);
     tnext = (p < Pi - 1) ? Y0[(p + 1) * 64 + lane] : T(0);
+    __syncthreads();   // the group's LDS may be rewritten by its next call
   }
   // phase 3: re-solve the segment with its end couplings on the right-hand side
   if (len > 0) {
@@ -373,22 +372,105 @@ __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt,
   }
 }
 
-// ---------------------------------------------------------------------------
-// Fused residual + volume-weighted restriction (mgMK/mg_restr.F90:124-164):
-// thread per coarse cell in natural (I fastest) order, so the fine reads of
-// a wave are contiguous half-rows; the fine residual is never stored.
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict__ Lt,
-                                                     const DevXfer* __restrict__ X, LaunchList ll,
-                                                     const int* __restrict__ active) {
+template <class T, int S>
+__global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                int parity, int zero_guess) {
+  extern __shared__ unsigned char lds_raw[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevXfer x = X[ll.items[t]];
-  const DevLevel& F = Lt[x.fine];
-  const DevLevel& C = Lt[x.coarse];
-  if (active && !active[F.inst]) return;
+  const DevLevel& L = Lt[ll.items[t]];
+  zline_group<T, S>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, threadIdx.y,
+                    threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
+}
+
+template <class T> struct TailNz;
+template <> struct TailNz<float> { static constexpr int value = kTailNz; };
+template <> struct TailNz<double> { static constexpr int value = kTailNz64; };
+
+// Short columns (nz <= TailNz) inside the coarse tail: one thread per
+// column, the whole column in registers, plain Thomas (the reference's line
+// solve, mgMK/mg_relax.F90:541-595); column index col = j*hx + m.
+template <class T, int NZ>
+__device__ __forceinline__ void zline_column(const DevLevel& L, int parity, int zero_guess, int col) {
+  const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
+  const int m = col % hx, j = col / hx;
+  const int c = parity, oth = parity ^ 1;
+  const int i = 2 * m + ((c + j) & 1), r = i & 1;
+  int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
+  int mr = m + r;     if (mr >= hx) mr = 0;
+  const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+  const T* __restrict__ co = coefs<T>(L);
+  const T* cy = co + 3 * nx;
+  const T* cz = cy + 3 * ny;
+  const T wxl = co[i], wxr = co[nx + i], wys = cy[j], wyn = cy[ny + j];
+  const T dxy = co[2 * nx + i] + cy[2 * ny + j];
+  T* __restrict__ u = static_cast<T*>(L.u);
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const size_t plane = (size_t)ny * 2 * hx;
+  const size_t own = cidx(m, j, 0, c, hx, ny);
+  const size_t xo = cidx(0, j, 0, oth, hx, ny);
+  const size_t so = cidx(m, jm, 0, oth, hx, ny);
+  const size_t no = cidx(m, jp, 0, oth, hx, ny);
+  const bool pz = L.pz != 0;
+  const bool lag = pz && !(zero_guess & 2);
+  const T lagb = lag ? cz[0] * u[(size_t)(nz - 1) * plane + own] : T(0);
+  const T lagt = lag ? cz[2 * nz - 1] * u[own] : T(0);
+  T g[NZ], cpa[NZ];
+#pragma unroll
+  for (int q = 0; q < NZ; ++q) {
+    const int k = min(q, nz - 1);
+    const size_t kb = (size_t)k * plane;
+    T v = f[kb + own];
+    if (!(zero_guess & 1)) {
+      T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
+      s += wys * u[kb + so];
+      s += wyn * u[kb + no];
+      v = v - s;
+    }
+    v -= (q == 0) ? lagb : T(0);
+    v -= (q == nz - 1) ? lagt : T(0);
+    g[q] = v;
+  }
+  T cp = T(0), yf = T(0);
+#pragma unroll
+  for (int q = 0; q < NZ; ++q) {
+    if (q < nz) {
+      const T a = (q == 0) ? T(0) : cz[q];
+      const T e = (q == nz - 1) ? T(0) : cz[nz + q];
+      const T rd = rcp((cz[2 * nz + q] + dxy) - a * cp);
+      yf = (g[q] - a * yf) * rd;
+      cp = e * rd;
+      g[q] = yf;
+      cpa[q] = cp;
+    }
+  }
+  T x = T(0);
+#pragma unroll
+  for (int q = NZ - 1; q >= 0; --q) {
+    if (q < nz) {
+      x = g[q] - cpa[q] * x;
+      u[(size_t)q * plane + own] = x;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_zline_col(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                      int parity, int zero_guess) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevLevel& L = Lt[ll.items[t]];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e >= C.ncell) return;
+  if (e < L.hx * L.ny) zline_column<T, TailNz<T>::value>(L, parity, zero_guess, e);
+}
+
+// ---------------------------------------------------------------------------
+// Fused residual + volume-weighted restriction (mgMK/mg_restr.F90:124-164):
+// element e is a coarse cell in natural (I fastest) order, so the fine reads
+// of a wave are contiguous half-rows; the fine residual is never stored.
+template <class T>
+__device__ __forceinline__ void restrict_elem(const DevLevel& F, const DevLevel& C, const DevXfer& x,
+                                              int e) {
   const int I = e % C.nx, JK = e / C.nx, J = JK % C.ny, K = JK / C.ny;
   const T* __restrict__ co = coefs<T>(F);
   const T* __restrict__ u = static_cast<const T*>(F.u);
@@ -416,6 +498,17 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
   fc[cidx(I >> 1, J, K, (I + J) & 1, C.hx, C.ny)] = acc / vs;
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict__ Lt,
+                                                     const DevXfer* __restrict__ X, LaunchList ll) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& C = Lt[x.coarse];
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e < C.ncell) restrict_elem<T>(Lt[x.fine], C, x, e);
+}
+
 // 1-D prolongation weights of fine index i: coarse (I0, w0), (I1, w1).
 __device__ __forceinline__ void pw(int i, int nc, int fac, int per, int bc0, int bc1, int& I0,
                                    int& I1, double& w0, double& w1) {
@@ -435,20 +528,11 @@ __device__ __forceinline__ void pw(int i, int nc, int fac, int per, int bc0, int
 // ---------------------------------------------------------------------------
 // Trilinear prolongation-add (mgMK/mg_cor.F90:102-159): fine u += sum of
 // 27,9,9,3,9,3,3,1 /64 of the eight nearest coarse cells; coarse ghosts are
-// periodic wraps, Neumann copies or Dirichlet zeros.  Thread per fine cell
-// in chequer order (contiguous read-modify-write of the fine field).
+// periodic wraps, Neumann copies or Dirichlet zeros.  Element e is a fine
+// cell in chequer order (contiguous read-modify-write of the fine field).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__ Lt,
-                                                    const DevXfer* __restrict__ X, LaunchList ll,
-                                                    const int* __restrict__ active) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevXfer x = X[ll.items[t]];
-  const DevLevel& F = Lt[x.fine];
-  const DevLevel& C = Lt[x.coarse];
-  if (active && !active[F.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e >= F.ncell) return;
+__device__ __forceinline__ void prolong_elem(const DevLevel& F, const DevLevel& C, const DevXfer& x,
+                                             int e) {
   int i, j, k;
   decode(F, e, i, j, k);
   int I[2], J[2], K[2];
@@ -462,11 +546,11 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
   for (int a = 0; a < 2; ++a) {
     T sy = T(0);
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const T v0 = uc[cidx(I[0] >> 1, J[b], K[a], (I[0] + J[b]) & 1, C.hx, C.ny)];
-      const T v1 = uc[cidx(I[1] >> 1, J[b], K[a], (I[1] + J[b]) & 1, C.hx, C.ny)];
+    for (int q = 0; q < 2; ++q) {
+      const T v0 = uc[cidx(I[0] >> 1, J[q], K[a], (I[0] + J[q]) & 1, C.hx, C.ny)];
+      const T v1 = uc[cidx(I[1] >> 1, J[q], K[a], (I[1] + J[q]) & 1, C.hx, C.ny)];
       const T sx = T(wx[0]) * v0 + T(wx[1]) * v1;
-      sy += T(wy[b]) * sx;
+      sy += T(wy[q]) * sx;
     }
     acc += T(wz[a]) * sy;
   }
@@ -474,131 +558,238 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
   u[e] += acc;
 }
 
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__ Lt,
+                                                    const DevXfer* __restrict__ X, LaunchList ll) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& F = Lt[x.fine];
+  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
+  if (e < F.ncell) prolong_elem<T>(F, Lt[x.coarse], x, e);
+}
+
 // ---------------------------------------------------------------------------
 // Coarsest level: u = Minv f with a precomputed FP64 (pseudo-)inverse
 // (replaces the ipre+ipost sweeps of mgMK/mg_cycle.F90:117-148).  Minv is
-// stored transposed so thread t's reads Minv^T[s][t] are coalesced.
+// stored transposed so the reads Minv^T[s][r] of consecutive rows r coalesce.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                   const int* __restrict__ active) {
+__device__ __forceinline__ void coarse_row(const DevLevel& L, int r) {
+  const int n = L.ncell;
+  const T* __restrict__ f = static_cast<const T*>(L.f);
+  const double* __restrict__ M = L.minvT + r;
+  // 8 independent partial sums so that the loads of a row are in flight together
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int s = 0;
+  for (; s + 8 <= n; s += 8) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] += M[(size_t)(s + q) * n] * (double)f[s + q];
+  }
+  for (; s < n; ++s) acc[0] += M[(size_t)s * n] * (double)f[s];
+  const double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  static_cast<T*>(L.u)[r] = (T)sum;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ Lt, LaunchList ll) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
   const int r = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  const int n = L.ncell;
-  if (r >= n) return;
-  const T* __restrict__ f = static_cast<const T*>(L.f);
-  const double* __restrict__ M = L.minvT;
-  double acc = 0.0;
-  for (int s = 0; s < n; ++s) acc += M[(size_t)s * n + r] * (double)f[s];
-  static_cast<T*>(L.u)[r] = (T)acc;
+  if (r < L.ncell) coarse_row<T>(L, r);
+}
+
+// ---------------------------------------------------------------------------
+// Coarse tail: the whole sub-V-cycle below a level (pre-smoothing and
+// residual restriction down to the coarsest level, its exact solve, then
+// prolongation and post-smoothing back up -- the order of
+// mgMK/mg_cycle.F90) for one instance in one workgroup of 64*kTailWaves
+// threads, with workgroup barriers between the steps instead of dependent
+// launches.  The tail levels' u and f live in LDS (the first level's f is
+// copied in, its u copied out); the element routines are the same as in the
+// standalone kernels, reached through level descriptors whose field
+// pointers point into LDS.  One workgroup per item (DevTail).
+template <class T>
+__global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
+                                                          const DevXfer* __restrict__ X,
+                                                          const DevTail* __restrict__ Tt,
+                                                          LaunchList ll) {
+  extern __shared__ unsigned char lds_raw[];
+  __shared__ DevLevel Ls[kTailMaxLevels];
+  const DevTail& tl = Tt[ll.items[blockIdx.x]];
+  const int tid = threadIdx.y * 64 + threadIdx.x, nthr = 64 * blockDim.y;
+  const int nl = tl.nlev;
+  T* buf = reinterpret_cast<T*>(lds_raw);
+  if (tid < nl) {
+    DevLevel d = Lt[tl.lev[tid]];
+    d.u = buf + tl.lds_u[tid];
+    d.f = buf + tl.lds_f[tid];
+    Ls[tid] = d;
+  }
+  const DevLevel& G0 = Lt[tl.lev[0]];
+  {
+    const T* gf = static_cast<const T*>(G0.f);
+    const T* gu = static_cast<const T*>(G0.u);
+    for (int e = tid; e < G0.ncell; e += nthr) {
+      buf[tl.lds_f[0] + e] = gf[e];
+      if (tl.copy_u_in) buf[tl.lds_u[0] + e] = gu[e];
+    }
+  }
+  __syncthreads();
+  auto half = [&](const DevLevel& L, int parity, int zg) {
+    if (tl.zline) {
+      const int n = L.hx * L.ny;
+      for (int e = tid; e < n; e += nthr) zline_column<T, TailNz<T>::value>(L, parity, zg, e);
+    } else {
+      const int n = L.hx * L.ny * L.nz;
+      for (int e = tid; e < n; e += nthr) rb_point_elem<T>(L, e, parity, zg);
+    }
+    __syncthreads();
+  };
+  for (int d = 0; d + 1 < nl; ++d) {
+    const DevLevel& L = Ls[d];
+    const bool zf = d > 0 || tl.zero_first;
+    if (zf && tl.pre[d] == 0) {
+      for (int e = tid; e < L.ncell; e += nthr) static_cast<T*>(L.u)[e] = T(0);
+      __syncthreads();
+    }
+    for (int s = 0; s < tl.pre[d]; ++s) {
+      half(L, 0, (zf && s == 0) ? 3 : 0);
+      half(L, 1, (zf && s == 0) ? 2 : 0);
+    }
+    const DevXfer x = X[tl.xf[d]];
+    const DevLevel& C = Ls[d + 1];
+    for (int e = tid; e < C.ncell; e += nthr) restrict_elem<T>(L, C, x, e);
+    __syncthreads();
+  }
+  {
+    const DevLevel& L = Ls[nl - 1];
+    for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, r);
+    __syncthreads();
+  }
+  for (int d = nl - 2; d >= 0; --d) {
+    const DevLevel& L = Ls[d];
+    const DevXfer x = X[tl.xf[d]];
+    for (int e = tid; e < L.ncell; e += nthr) prolong_elem<T>(L, Ls[d + 1], x, e);
+    __syncthreads();
+    for (int s = 0; s < tl.post[d]; ++s) {
+      half(L, 0, 0);
+      half(L, 1, 0);
+    }
+  }
+  T* gu = static_cast<T*>(G0.u);
+  for (int e = tid; e < G0.ncell; e += nthr) gu[e] = buf[tl.lds_u[0] + e];
 }
 
 // ---------------------------------------------------------------------------
 // FP64 finest-level kernels.
+// Each thread handles kFinePer cells e = e0 + s*kBlock (coalesced), keeps
+// running maxima, and the block writes one partial per value; the finalize
+// kernel folds the partials per item.
+template <int NV, class F>
+__device__ __forceinline__ void fine_loop(const DevLevel& L, int lb, double* __restrict__ partial,
+                                          int b, F&& body) {
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  const int e0 = lb * kBlock * kFinePer + threadIdx.x;
+#pragma unroll
+  for (int s = 0; s < kFinePer; ++s) {
+    const int e = e0 + s * kBlock;
+    if (e < L.ncell) body(e, v);
+  }
+  block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
+}
+
+// {max|b - A u|, max|b|}
 __global__ __launch_bounds__(kBlock) void k_residual_max(const DevLevel* __restrict__ Lt,
-                                                         LaunchList ll, double* __restrict__ partial,
-                                                         const int* __restrict__ active) {
+                                                         LaunchList ll,
+                                                         double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  double v[2] = {0.0, 0.0};
-  if (e < L.ncell) {
+  const double* __restrict__ co = L.co64;
+  const double* __restrict__ u = static_cast<const double*>(L.u);
+  const double* __restrict__ f = static_cast<const double*>(L.f);
+  fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
     int i, j, k;
     decode(L, e, i, j, k);
-    const double* __restrict__ co = L.co64;
-    const double* __restrict__ u = static_cast<const double*>(L.u);
-    const double* __restrict__ f = static_cast<const double*>(L.f);
     const Nb n = neighbours(L, i, j, k);
     const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
-    v[0] = fabs(f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]));
-    v[1] = fabs(f[n.c]);
-  }
-  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
+    v[0] = nmax(v[0], fabs(f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c])));
+    v[1] = nmax(v[1], fabs(f[n.c]));
+  });
 }
 
+// {max|u - u2|, max|u|}
 __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restrict__ Lt,
-                                                       LaunchList ll, double* __restrict__ partial,
-                                                       const int* __restrict__ active) {
+                                                       LaunchList ll, double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  double v[2] = {0.0, 0.0};
-  if (e < L.ncell) {
-    const double a = static_cast<const double*>(L.u)[e];
-    const double b = static_cast<const double*>(L.u2)[e];
-    v[0] = fabs(a - b);
-    v[1] = fabs(a);
-  }
-  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
+  const double* __restrict__ u = static_cast<const double*>(L.u);
+  const double* __restrict__ u2 = static_cast<const double*>(L.u2);
+  fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
+    v[0] = nmax(v[0], fabs(u[e] - u2[e]));
+    v[1] = nmax(v[1], fabs(u[e]));
+  });
 }
 
+// r = b - A u (FP64) -> FP32 right-hand side of the correction level; {max|r|, max|b|}
 __global__ __launch_bounds__(kBlock) void k_defect32(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                     double* __restrict__ partial,
-                                                     const int* __restrict__ active) {
+                                                     double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  double v[2] = {0.0, 0.0};
-  if (e < L.ncell) {
+  const double* __restrict__ co = L.co64;
+  const double* __restrict__ u = static_cast<const double*>(L.u);
+  const double* __restrict__ f = static_cast<const double*>(L.f);
+  float* __restrict__ r32 = static_cast<float*>(Lt[L.link].f);
+  fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
     int i, j, k;
     decode(L, e, i, j, k);
-    const double* __restrict__ co = L.co64;
-    const double* __restrict__ u = static_cast<const double*>(L.u);
-    const double* __restrict__ f = static_cast<const double*>(L.f);
     const Nb n = neighbours(L, i, j, k);
     const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
     const double r = f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]);
-    static_cast<float*>(Lt[L.link].f)[e] = (float)r;
-    v[0] = fabs(r);
-    v[1] = fabs(f[n.c]);
-  }
-  block_reduce_store<2, false>(v, partial + (size_t)b * 2);
+    r32[e] = (float)r;
+    v[0] = nmax(v[0], fabs(r));
+    v[1] = nmax(v[1], fabs(f[n.c]));
+  });
 }
 
 // p_out = p_in + e, then r = b - A p_out from p_in and e at the seven points
-// (p_in is never written, so neighbouring threads read consistent values).
+// (p_in is never written, so neighbouring threads read consistent values);
+// {max|r|, max|e|, max|p_out|}
 __global__ __launch_bounds__(kBlock) void k_correct_defect32(const DevLevel* __restrict__ Lt,
                                                              LaunchList ll,
-                                                             double* __restrict__ partial,
-                                                             const int* __restrict__ active) {
+                                                             double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  if (active && !active[L.inst]) return;
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  double v[3] = {0.0, 0.0, 0.0};
-  if (e < L.ncell) {
+  const DevLevel& S = Lt[L.link];
+  const double* __restrict__ co = L.co64;
+  const double* __restrict__ p = static_cast<const double*>(L.u);
+  const float* __restrict__ ev = static_cast<const float*>(S.u);
+  const double* __restrict__ bb = static_cast<const double*>(L.f);
+  double* __restrict__ pout = static_cast<double*>(L.u2);
+  float* __restrict__ r32 = static_cast<float*>(S.f);
+  fine_loop<3>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
     int i, j, k;
     decode(L, e, i, j, k);
-    const DevLevel& S = Lt[L.link];
-    const double* __restrict__ co = L.co64;
-    const double* __restrict__ p = static_cast<const double*>(L.u);
-    const float* __restrict__ ev = static_cast<const float*>(S.u);
-    const double* __restrict__ b = static_cast<const double*>(L.f);
     const Nb n = neighbours(L, i, j, k);
 #define PN(q) (p[n.q] + (double)ev[n.q])
     const double pc = PN(c);
     const double s = offsum<double>(co, L, i, j, k, PN(xm), PN(xp), PN(ym), PN(yp), PN(zm), PN(zp));
 #undef PN
-    const double r = b[n.c] - (s + diag_at<double>(co, L, i, j, k) * pc);
-    static_cast<double*>(L.u2)[n.c] = pc;
-    static_cast<float*>(S.f)[n.c] = (float)r;
-    v[0] = fabs(r);
-    v[1] = fabs((double)ev[n.c]);
-    v[2] = fabs(pc);
-  }
-  block_reduce_store<3, false>(v, partial + (size_t)b * 3);
+    const double r = bb[n.c] - (s + diag_at<double>(co, L, i, j, k) * pc);
+    pout[n.c] = pc;
+    r32[n.c] = (float)r;
+    v[0] = nmax(v[0], fabs(r));
+    v[1] = nmax(v[1], fabs((double)ev[n.c]));
+    v[2] = nmax(v[2], fabs(pc));
+  });
 }
 
-// Per-item reduction of block partials (fixed order: deterministic).
 // One block per item: thread t folds partials b0+t, b0+t+256, ... in order,
 // then a fixed-shape tree combines the threads (deterministic).
 __global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
@@ -650,22 +841,32 @@ __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restr
   static_cast<double*>(which == 0 ? L.u : L.f)[e] = v;
 }
 
+// Weighted sums: each thread folds kWsumPer cells (in a fixed order), one
+// partial per block.
 __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                  int which, double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
+  const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
   double v[1] = {0.0};
-  if (e < L.ncell) {
-    int i, j, k;
-    decode(L, e, i, j, k);
-    const double w = L.vol[L.nx + L.ny + k] * L.vol[L.nx + j] * L.vol[i];
-    const double x = (which == 2) ? L.rho[k]
-                                  : static_cast<const double*>(which == 0 ? L.u : L.f)[e];
-    v[0] = w * x;
+  for (int s = 0; s < kWsumPer; ++s) {
+    const int e = e0 + s * kBlock;
+    if (e < L.ncell) {
+      int i, j, k;
+      decode(L, e, i, j, k);
+      const double w = L.vol[L.nx + L.ny + k] * L.vol[L.nx + j] * L.vol[i];
+      const double x = (which == 2) ? L.rho[k]
+                                    : static_cast<const double*>(which == 0 ? L.u : L.f)[e];
+      v[0] += w * x;
+    }
   }
   block_reduce_store<1, true>(v, partial + b);
+}
+
+__global__ void k_zero(double* __restrict__ a, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = 0.0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ Lt, LaunchList ll,
@@ -885,9 +1086,9 @@ inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) /
 #define LAUNCH(kern, grid, blk, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, s, __VA_ARGS__)
 
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
-                     const int* active, hipStream_t s) {
-  if (fp64) LAUNCH(k_rb_point<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess, active);
-  else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess, active);
+                     hipStream_t s) {
+  if (fp64) LAUNCH(k_rb_point<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
+  else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
 }
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                   int segs, hipStream_t s) {
@@ -898,36 +1099,68 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
   else
     hipLaunchKernelGGL((k_zline<float, kZSeg>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess);
 }
+void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                      int zero_guess, hipStream_t s) {
+  if (fp64) LAUNCH(k_zline_col<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
+  else LAUNCH(k_zline_col<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
+}
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                     const int* active, hipStream_t s) {
-  if (fp64) LAUNCH(k_restrict<double>, ll.nblocks, kBlock, L, X, ll, active);
-  else LAUNCH(k_restrict<float>, ll.nblocks, kBlock, L, X, ll, active);
+                     hipStream_t s) {
+  if (fp64) LAUNCH(k_restrict<double>, ll.nblocks, kBlock, L, X, ll);
+  else LAUNCH(k_restrict<float>, ll.nblocks, kBlock, L, X, ll);
 }
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                    const int* active, hipStream_t s) {
-  if (fp64) LAUNCH(k_prolong<double>, ll.nblocks, kBlock, L, X, ll, active);
-  else LAUNCH(k_prolong<float>, ll.nblocks, kBlock, L, X, ll, active);
+                    hipStream_t s) {
+  if (fp64) LAUNCH(k_prolong<double>, ll.nblocks, kBlock, L, X, ll);
+  else LAUNCH(k_prolong<float>, ll.nblocks, kBlock, L, X, ll);
 }
-void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, const int* active,
-                   hipStream_t s) {
-  if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll, active);
-  else LAUNCH(k_coarse<float>, ll.nblocks, kBlock, L, ll, active);
+void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s) {
+  if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll);
+  else LAUNCH(k_coarse<float>, ll.nblocks, kBlock, L, ll);
 }
-void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial,
-                         const int* active, hipStream_t s) {
-  LAUNCH(k_residual_max, ll.nblocks, kBlock, L, ll, partial, active);
+void launch_tail(bool fp64, const DevLevel* L, const DevXfer* X, const DevTail* T,
+                 const LaunchList& ll, size_t lds_bytes, hipStream_t s) {
+  const dim3 blk(64, kTailWaves);
+  if (fp64) hipLaunchKernelGGL(k_tail<double>, dim3(ll.nitems), blk, lds_bytes, s, L, X, T, ll);
+  else hipLaunchKernelGGL(k_tail<float>, dim3(ll.nitems), blk, lds_bytes, s, L, X, T, ll);
 }
-void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, const int* active,
-                       hipStream_t s) {
-  LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial, active);
+
+size_t tail_lds_limit() {
+  static size_t limit = 0;
+  if (limit) return limit;
+  int dev = 0, v = 0;
+  size_t lim = 64 * 1024;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      v > 0)
+    lim = (size_t)v;
+  // room for the static descriptor array of the kernel
+  lim -= sizeof(DevLevel) * kTailMaxLevels + 256;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<double>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess ||
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<float>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
+    (void)hipGetLastError();
+    lim = 64 * 1024 - sizeof(DevLevel) * kTailMaxLevels - 256;
+  }
+  limit = lim;
+  return limit;
 }
-void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, const int* active,
-                     hipStream_t s) {
-  LAUNCH(k_defect32, ll.nblocks, kBlock, L, ll, partial, active);
+void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
+  LAUNCH(k_residual_max, ll.nblocks, kBlock, L, ll, partial);
+}
+void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
+  LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial);
+}
+void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
+  LAUNCH(k_defect32, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
-                             const int* active, hipStream_t s) {
-  LAUNCH(k_correct_defect32, ll.nblocks, kBlock, L, ll, partial, active);
+                             hipStream_t s) {
+  LAUNCH(k_correct_defect32, ll.nblocks, kBlock, L, ll, partial);
+}
+void launch_zero(double* a, int n, hipStream_t s) {
+  LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
 }
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
                      int ostride, hipStream_t s) {

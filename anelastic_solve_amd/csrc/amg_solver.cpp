@@ -100,6 +100,7 @@ Context::~Context() {
   inst.clear();
   if (dL) (void)hipFree(dL);
   if (dX) (void)hipFree(dX);
+  if (dT) (void)hipFree(dT);
   if (d_partial) (void)hipFree(d_partial);
   if (d_out) (void)hipFree(d_out);
   if (h_out) (void)hipHostFree(h_out);
@@ -223,6 +224,48 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     I->xf.push_back((int)hX.size());
     hX.push_back(x);
   }
+  // coarse tail: every level from the first one with <= kTailCells cells
+  {
+    // grow the tail upward from the coarsest level while it stays small,
+    // short-columned and inside the LDS budget
+    const size_t limit = tail_lds_limit();
+    int l0 = L - 1;
+    size_t bytes = 2 * (size_t)I->H.lv[L - 1].ncell() * w;
+    for (int l = L - 2; l >= 0; --l) {
+      const HostLevel& H = I->H.lv[l];
+      const size_t nb2 = bytes + 2 * (size_t)H.ncell() * w;
+      if (H.ncell() > kTailCells || nb2 > limit) break;
+      if (cfg.smoother == AMG_SMOOTHER_LJ && H.nz > (I->fp64 ? kTailNz64 : kTailNz)) break;
+      l0 = l;
+      bytes = nb2;
+    }
+    if (bytes > limit) {
+      set_error("amg_create: the coarsest level does not fit in LDS");
+      delete I;
+      return AMG_EBADSIZE;
+    }
+    I->l0 = l0;
+    DevTail tl;
+    std::memset(&tl, 0, sizeof tl);
+    tl.nlev = L - l0;
+    tl.zero_first = (l0 > 0 || !I->fp64) ? 1 : 0;
+    tl.copy_u_in = tl.zero_first ? 0 : 1;
+    tl.zline = cfg.smoother == AMG_SMOOTHER_LJ ? 1 : 0;
+    int off = 0;
+    for (int d = 0; d < tl.nlev; ++d) {
+      tl.lev[d] = I->lev[l0 + d];
+      if (l0 + d < L - 1) tl.xf[d] = I->xf[l0 + d];
+      tl.pre[d] = I->pre[l0 + d];
+      tl.post[d] = I->post[l0 + d];
+      const int nc = (int)I->H.lv[l0 + d].ncell();
+      tl.lds_u[d] = off;
+      tl.lds_f[d] = off + nc;
+      off += 2 * nc;
+    }
+    I->tail = (int)hT.size();
+    I->tail_lds = (size_t)off * w;
+    hT.push_back(tl);
+  }
   // projection metrics
   {
     const HostLevel& F = I->H.lv[0];
@@ -258,6 +301,12 @@ int Context::upload_tables() {
     capX = std::max(hX.size() * 2, (size_t)16);
     AMG_HIP(hipMalloc((void**)&dX, capX * sizeof(DevXfer)));
   }
+  if (hT.size() > capT || !dT) {
+    if (dT) AMG_HIP(hipFree(dT));
+    capT = std::max(hT.size() * 2, (size_t)16);
+    AMG_HIP(hipMalloc((void**)&dT, capT * sizeof(DevTail)));
+  }
+  AMG_HIP(hipMemcpy(dT, hT.data(), hT.size() * sizeof(DevTail), hipMemcpyHostToDevice));
   AMG_HIP(hipMemcpy(dL, hL.data(), hL.size() * sizeof(DevLevel), hipMemcpyHostToDevice));
   if (!hX.empty())
     AMG_HIP(hipMemcpy(dX, hX.data(), hX.size() * sizeof(DevXfer), hipMemcpyHostToDevice));
@@ -294,13 +343,13 @@ void Context::zline_launches(const std::vector<Item>& items, bool fp64, int pari
 void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   if (S.empty()) return;
   const bool zline = cfg.smoother == AMG_SMOOTHER_LJ;
-  int maxL = 0;
-  for (Instance* I : S) maxL = std::max(maxL, (int)I->lev.size());
+  int maxL0 = 0;   // levels above the coarse tail, cycled with one launch per step
+  for (Instance* I : S) maxL0 = std::max(maxL0, I->l0);
   const DevLevel* L = dL;
   auto half = [&](const std::vector<Item>& items, int parity, int zg) {
     if (zline) zline_launches(items, fp64, parity, zg);
     else for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_rb_point(fp64, L, ll, parity, zg, nullptr, stream);
+      launch_rb_point(fp64, L, ll, parity, zg, stream);
     });
   };
   auto blocks_smooth = [&](const DevLevel& d) {
@@ -309,11 +358,11 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   auto smooth = [&](int dpt, bool pre, bool zero_first) {
     int maxs = 0;
     for (Instance* I : S)
-      if (dpt < (int)I->lev.size() - 1) maxs = std::max(maxs, pre ? I->pre[dpt] : I->post[dpt]);
+      if (dpt < I->l0) maxs = std::max(maxs, pre ? I->pre[dpt] : I->post[dpt]);
     for (int s = 0; s < maxs; ++s) {
       std::vector<Item> items;
       for (Instance* I : S) {
-        if (dpt >= (int)I->lev.size() - 1) continue;
+        if (dpt >= I->l0) continue;
         if (s >= (pre ? I->pre[dpt] : I->post[dpt])) continue;
         const int li = I->lev[dpt];
         items.push_back({li, blocks_smooth(hL[li])});
@@ -324,48 +373,50 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
       half(items, 1, (zero_first && s == 0) ? 2 : 0);
     }
   };
-  for (int dpt = 0; dpt + 1 < maxL; ++dpt) {
+  for (int dpt = 0; dpt < maxL0; ++dpt) {
     const bool zero_first = (dpt > 0) || !fp64;
     std::vector<Item> zero;
     for (Instance* I : S)
-      if (dpt < (int)I->lev.size() - 1 && zero_first && I->pre[dpt] == 0)
+      if (dpt < I->l0 && zero_first && I->pre[dpt] == 0)
         zero.push_back({I->lev[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
     for_lists(zero, [&](const LaunchList& ll, int, int) { launch_fill_zero(fp64, L, ll, stream); });
     smooth(dpt, true, zero_first);
     std::vector<Item> items;
     for (Instance* I : S)
-      if (dpt < (int)I->lev.size() - 1)
-        items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt + 1]].ncell, kBlock)});
+      if (dpt < I->l0) items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt + 1]].ncell, kBlock)});
     for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_restrict(fp64, L, dX, ll, nullptr, stream);
+      launch_restrict(fp64, L, dX, ll, stream);
     });
   }
   {
     std::vector<Item> items;
+    size_t lds = 0;
     for (Instance* I : S) {
-      const int li = I->lev.back();
-      items.push_back({li, cdiv(hL[li].ncell, kBlock)});
+      items.push_back({I->tail, 1});
+      lds = std::max(lds, I->tail_lds);
     }
-    for_lists(items, [&](const LaunchList& ll, int, int) { launch_coarse(fp64, L, ll, nullptr, stream); });
+    for_lists(items, [&](const LaunchList& ll, int, int) {
+      launch_tail(fp64, L, dX, dT, ll, lds, stream);
+    });
   }
-  for (int dpt = maxL - 2; dpt >= 0; --dpt) {
+  for (int dpt = maxL0 - 1; dpt >= 0; --dpt) {
     std::vector<Item> items;
     for (Instance* I : S)
-      if (dpt < (int)I->lev.size() - 1)
-        items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
+      if (dpt < I->l0) items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
     for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_prolong(fp64, L, dX, ll, nullptr, stream);
+      launch_prolong(fp64, L, dX, ll, stream);
     });
     smooth(dpt, false, false);
   }
 }
 
 namespace {
-std::vector<Item> fine_items(const Context& C, const std::vector<Instance*>& S, bool current) {
+std::vector<Item> fine_items(const Context& C, const std::vector<Instance*>& S, bool current,
+                             long long per_block = kBlock) {
   std::vector<Item> items;
   for (Instance* I : S) {
     const int li = current ? I->fine[I->cur] : I->fine[0];
-    items.push_back({li, cdiv(C.hL[li].ncell, kBlock)});
+    items.push_back({li, cdiv(C.hL[li].ncell, per_block)});
   }
   return items;
 }
@@ -382,20 +433,19 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S) {
     for_lists(items, [&](const LaunchList& ll, int, int) { launch_copy_u(dL, ll, nullptr, stream); });
   }
   vcycle(S, fp64);
-  auto items = fine_items(*this, S, true);
+  auto items = fine_items(*this, S, true, (long long)kBlock * kFinePer);
   const int nv = 4;
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
     if (fp64) {
-      launch_residual_max(dL, ll, part, nullptr, stream);
+      launch_residual_max(dL, ll, part, stream);
       launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
       if (relmax) {
-        launch_change_max(dL, ll, part, nullptr, stream);
-        // stored at value slots 2,3
+        launch_change_max(dL, ll, part, stream);
         launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
       }
     } else {
-      launch_correct_defect32(dL, ll, part, nullptr, stream);
+      launch_correct_defect32(dL, ll, part, stream);
       launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream);
     }
   });
@@ -450,6 +500,28 @@ void Context::post_norms(const std::vector<Instance*>& S) {
   }
 }
 
+// Singular instances: rhs == true removes the incompatible part of b
+// (b -= sum V b / sum V); otherwise removes the null-space part of the
+// current pressure (p -= (sum V p / sum V rho) rho).
+void Context::remove_mean(const std::vector<Instance*>& S, bool rhs) {
+  std::vector<Item> sums, shifts;
+  for (Instance* I : S) {
+    if (!I->H.singular) continue;
+    const int li = rhs ? I->fine[0] : I->fine[I->cur];
+    sums.push_back({li, cdiv(hL[li].ncell, (long long)kBlock * kWsumPer)});
+    shifts.push_back({li, cdiv(hL[li].ncell, kBlock)});
+  }
+  if (sums.empty()) return;
+  double* c = d_out + kOutRegion * 3;
+  for_lists(sums, [&](const LaunchList& ll, int ioff, int boff) {
+    launch_wsum(dL, ll, rhs ? 1 : 0, d_partial + boff, stream);
+    launch_finalize(ll, d_partial + boff, 1, 1, c + ioff, 1, stream);
+  });
+  for_lists(shifts, [&](const LaunchList& ll, int ioff, int) {
+    launch_shift(dL, ll, rhs ? 1 : 0, c + ioff, rhs ? 0 : 1, stream);
+  });
+}
+
 int Context::prepare(const std::vector<Instance*>& S) {
   const int nv = 4;
   PtrList rhs{}, phi{};
@@ -473,21 +545,14 @@ int Context::prepare(const std::vector<Instance*>& S) {
     launch_from_fortran(dL, ll, phi, 0, 0, stream);
   });
   // singular: remove the incompatible part of b (sum V b = 0)
-  std::vector<Item> sing;
-  for (Instance* I : S)
-    if (I->H.singular) sing.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, kBlock)});
-  for_lists(sing, [&](const LaunchList& ll, int ioff, int boff) {
-    double* part = d_partial + (size_t)boff * nv;
-    launch_wsum(dL, ll, 1, part, stream);
-    launch_finalize(ll, part, 1, 1, d_out + kOutRegion * 3 + ioff, 1, stream);
-    launch_shift(dL, ll, 1, d_out + kOutRegion * 3 + ioff, 0, stream);
-  });
+  remove_mean(S, true);
   // initial residual (mixed: also the FP32 defect of the first cycle)
   const bool fp64 = S[0]->fp64;
-  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
+  const auto fitems = fine_items(*this, S, false, (long long)kBlock * kFinePer);
+  for_lists(fitems, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
-    if (fp64) launch_residual_max(dL, ll, part, nullptr, stream);
-    else launch_defect32(dL, ll, part, nullptr, stream);
+    if (fp64) launch_residual_max(dL, ll, part, stream);
+    else launch_defect32(dL, ll, part, stream);
     launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
   });
   AMG_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream));
@@ -511,20 +576,13 @@ int Context::prepare(const std::vector<Instance*>& S) {
 }
 
 int Context::finish(const std::vector<Instance*>& S) {
-  const int nv = 4;
-  std::vector<Item> cur, sing;
+  std::vector<Item> cur;
   for (Instance* I : S) {
     const int li = I->fine[I->cur];
     cur.push_back({li, cdiv(hL[li].ncell, kBlock)});
-    if (I->H.singular) sing.push_back({li, cdiv(hL[li].ncell, kBlock)});
   }
   // null-space removal: p -= (sum V p / sum V rho) rho
-  for_lists(sing, [&](const LaunchList& ll, int ioff, int boff) {
-    double* part = d_partial + (size_t)boff * nv;
-    launch_wsum(dL, ll, 0, part, stream);
-    launch_finalize(ll, part, 1, 1, d_out + kOutRegion * 3 + ioff, 1, stream);
-    launch_shift(dL, ll, 0, d_out + kOutRegion * 3 + ioff, 1, stream);
-  });
+  remove_mean(S, false);
   PtrList phi{};
   for_lists(cur, [&](const LaunchList& ll, int ioff, int) {
     for (int t = 0; t < ll.nitems; ++t) phi.p[t] = S[ioff + t]->d_phi;
@@ -648,8 +706,15 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     items.push_back({I->xf[level], cdiv(nc, kBlock)});
     b = (2.0 + 0.125) * w * d.ncell;
   } else if (which == 3) {
-    items.push_back({I->fine[I->cur], cdiv(hL[I->fine[I->cur]].ncell, kBlock)});
+    items.push_back({I->fine[I->cur], cdiv(hL[I->fine[I->cur]].ncell, (long long)kBlock * kFinePer)});
     b = fp64 ? 16.0 * d.ncell : 32.0 * d.ncell;
+  } else if (which == 7 || which == 8) {
+    if (!zline || d.nz > (fp64 ? kTailNz64 : kTailNz)) {
+      set_error("amg_run_kernel: the short-column z-line sweep needs the LJ smoother and a short nz");
+      return AMG_EINVAL;
+    }
+    items.push_back({I->lev[level], cdiv((long long)d.hx * d.ny, kBlock)});
+    b = 1.5 * w * d.ncell;
   } else if (which == 5) {
     if (level != L - 1) {
       set_error("amg_run_kernel: the coarse solve runs on the coarsest level");
@@ -666,17 +731,19 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
       if (which == 0 || which == 4 || which == 6) {
         const int par = which == 4 ? 1 : 0, zg = which == 6 ? 3 : 0;
         if (zline) zline_launches(items, fp64, par, zg);
-        else launch_rb_point(fp64, dL, ll, par, zg, nullptr, stream);
+        else launch_rb_point(fp64, dL, ll, par, zg, stream);
+      } else if (which == 7 || which == 8) {
+        launch_zline_col(fp64, dL, ll, which == 8 ? 1 : 0, 0, stream);
       } else if (which == 1) {
-        launch_restrict(fp64, dL, dX, ll, nullptr, stream);
+        launch_restrict(fp64, dL, dX, ll, stream);
       } else if (which == 2) {
-        launch_prolong(fp64, dL, dX, ll, nullptr, stream);
+        launch_prolong(fp64, dL, dX, ll, stream);
       } else if (which == 5) {
-        launch_coarse(fp64, dL, ll, nullptr, stream);
+        launch_coarse(fp64, dL, ll, stream);
       } else if (fp64) {
-        launch_residual_max(dL, ll, d_partial, nullptr, stream);
+        launch_residual_max(dL, ll, d_partial, stream);
       } else {
-        launch_correct_defect32(dL, ll, d_partial, nullptr, stream);
+        launch_correct_defect32(dL, ll, d_partial, stream);
       }
     });
   }

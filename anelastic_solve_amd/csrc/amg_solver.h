@@ -50,6 +50,9 @@ struct Instance {
   bool fp64 = false;             // V-cycle precision (true: plain FP64)
   std::vector<int> lev;          // DevLevel index per level, finest first
   std::vector<int> xf;           // DevXfer index per transition
+  int l0 = 0;                    // first level of the coarse tail
+  int tail = -1;                 // DevTail index
+  size_t tail_lds = 0;           // dynamic LDS bytes of the tail
   int fine[2] = {-1, -1};        // FP64 finest entries (u = p_cur for cur = 0/1)
   int cur = 0;                   // which fine entry holds the current pressure
   std::vector<int> pre, post;
@@ -74,9 +77,11 @@ struct Context {
   std::vector<Instance*> inst;
   std::vector<DevLevel> hL;
   std::vector<DevXfer> hX;
+  std::vector<DevTail> hT;
   DevLevel* dL = nullptr;
   DevXfer* dX = nullptr;
-  size_t capL = 0, capX = 0;
+  DevTail* dT = nullptr;
+  size_t capL = 0, capX = 0, capT = 0;
   double* d_partial = nullptr;   // block partials
   size_t cap_partial = 0;
   double* d_out = nullptr;       // per-item results
@@ -95,6 +100,7 @@ struct Context {
   void vcycle(const std::vector<Instance*>& S, bool fp64);
   void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
   int prepare(const std::vector<Instance*>& S);
+  void remove_mean(const std::vector<Instance*>& S, bool rhs);
   int finish(const std::vector<Instance*>& S);
   int cycle_and_norms(const std::vector<Instance*>& S);
   void post_norms(const std::vector<Instance*>& S);

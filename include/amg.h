@@ -223,6 +223,7 @@ int amg_server_drain(amg_server s);
  *         3: FP64 finest-level kernel (FP64: residual check; mixed: correct+defect)
  *         5: coarsest-level solve ('level' must be the coarsest)
  *         6: red half-sweep with a zero initial guess
+ *         7/8: red/black z-line half-sweep of the coarse tail (short columns)
  * Returns the kernel's algorithmic bytes per launch.  The caller times the
  * stream (amg_stream) with its own events. */
 int amg_run_kernel(amg_handle h, int which, int level, int precision, int reps, double* bytes);

@@ -36,11 +36,9 @@ def test_library_loads_and_exports_every_declared_symbol():
 
 
 def test_library_targets_gfx950():
-    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o",
-                          f"--input={_capi.LIB_PATH}"], capture_output=True, text=True)
-    if out.returncode != 0:
-        pytest.skip("offload bundler listing unavailable")
-    assert "gfx950" in out.stdout
+    """The library built here embeds a gfx950 code object (offload bundle target id)."""
+    blob = open(_capi.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
 
 
 def test_config_defaults():
