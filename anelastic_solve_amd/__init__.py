@@ -5,12 +5,14 @@ projection step around it; see ``include/amg.h`` for the C ABI and
 ``anelastic_solve_amd.mg`` for the reference-named Python interface.
 """
 from . import _capi
-from .grid import Axis, Grid, axis_from_faces, box_grid, load_grid_file, tanh_axis, uniform_axis
+from .grid import (Axis, Grid, axis_from_faces, box_grid, load_grid_dir, load_grid_file, tanh_axis,
+                   uniform_axis, write_grid_file)
 from .mg import (AMGError, BatchSolver, PressureSolver, Server, Pgrad, Psource, default_config,
                  divergence, mg_end, mg_init, mg_read_params, mg_solver)
 
 __all__ = [
-    "Axis", "Grid", "axis_from_faces", "box_grid", "load_grid_file", "tanh_axis", "uniform_axis",
+    "Axis", "Grid", "axis_from_faces", "box_grid", "load_grid_dir", "load_grid_file", "tanh_axis",
+    "uniform_axis", "write_grid_file",
     "AMGError", "BatchSolver", "PressureSolver", "Server", "Pgrad", "Psource", "default_config",
     "divergence", "mg_end", "mg_init", "mg_read_params", "mg_solver", "load_library",
 ]

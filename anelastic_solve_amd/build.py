@@ -17,6 +17,12 @@ SOURCES = ["amg_kernels.hip", "amg_hierarchy.cpp", "amg_solver.cpp", "amg_server
            "amg_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+# Fortran drop-in shim (ISO_C_BINDING) and its test driver, in link order
+FORTRAN = ["amg_cabi.F90", "amg_host_modules.F90", "amg_mg_shim.F90", "amg_shim_driver.F90"]
+FDIR = os.path.join(HERE, "fortran")
+FDRIVER = os.path.join(FDIR, "bin", "amg_shim_driver")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
@@ -33,7 +39,7 @@ def _run(cmd, verbose):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+        raise RuntimeError("build step failed: " + " ".join(cmd))
     return r
 
 
@@ -57,7 +63,30 @@ def build(verbose=False, jobs=4):
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+def build_fortran(verbose=False):
+    """Compile the Fortran shim and its driver against the library (skipped,
+    with a note, when the image has no flang)."""
+    if not os.path.exists(FLANG):
+        sys.stderr.write("flang not found: Fortran shim not built\n")
+        return None
+    moddir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(moddir, exist_ok=True)
+    os.makedirs(os.path.dirname(FDRIVER), exist_ok=True)
+    objs = []
+    for src in FORTRAN:
+        obj = os.path.join(moddir, os.path.splitext(src)[0] + ".o")
+        _run([FLANG, "-O2", "-fPIC", "-module-dir", moddir, "-c", os.path.join(FDIR, src),
+              "-o", obj], verbose)
+        objs.append(obj)
+    tmp = FDRIVER + ".tmp"
+    _run([FLANG, *objs, "-L" + HERE, "-lanelastic_mg", "-Wl,-rpath,$ORIGIN/../..", "-o", tmp],
+         verbose)
+    os.replace(tmp, FDRIVER)
+    return FDRIVER
 
 
 def _wait(item):

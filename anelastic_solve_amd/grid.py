@@ -109,6 +109,35 @@ def load_grid_file(path):
     return Axis(xe, xc, dxe, dxc), L, x0
 
 
+def write_grid_file(path, axis, L, x0, label="X1", gridtype="custom"):
+    """Write one direction in the format ``load_grid_file`` reads
+    (``utils/loadGrid.F90:44-58``): two comment lines, ``nxp2``, ``L``,
+    ``x0``, the grid type, then ``nxp2`` rows ``i xe xc dxe dxc rdxe rdxc``
+    with Fortran ``D`` exponents and a trailing spacing-ratio column."""
+    def d(v):
+        return ("%24.16E" % v).replace("E", "D")
+    n2 = axis.n + 2
+    out = [f"#{label}-GRID",
+           "#   i" + "".join(f"{h:>24}" for h in ("xe(i)", "xc(i)", "dxe(i)", "dxc(i)",
+                                                  "rdxe(i)", "rdxc(i)")) + " dxc(i)/dxc(i-1)",
+           f"{n2:6d}", d(L), d(x0), f" {gridtype}"]
+    for i in range(n2):
+        ratio = "n/a" if i == 0 else "%.8f" % (axis.dxc[i] / axis.dxc[i - 1])
+        out.append(f"{i + 1:5d}" + "".join(d(v) for v in (axis.xe[i], axis.xc[i], axis.dxe[i],
+                                                           axis.dxc[i], axis.rdxe[i],
+                                                           axis.rdxc[i])) + f"{ratio:>17}")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
+def load_grid_dir(directory, rho=None):
+    """``Grid`` from ``x1_grid.in``, ``x2_grid.in`` and ``x3_grid.in`` in ``directory``
+    (the three files ``utils/loadGrid.F90`` reads)."""
+    import os
+    axes = [load_grid_file(os.path.join(directory, f"x{d}_grid.in"))[0] for d in (1, 2, 3)]
+    return Grid(*axes, rho=rho)
+
+
 class Grid:
     """Three axes plus the background density ``rho(z)`` (length ``nz+2``, or None = 1)."""
 
