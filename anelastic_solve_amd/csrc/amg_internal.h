@@ -69,6 +69,18 @@ struct DevXfer {
 
 constexpr int kBlock = 256;
 
+// Prolongation runs one thread per dual cell: the (up to) 2x2x2 fine cells
+// between neighbouring coarse centres.  Per direction a factor-2 transfer
+// has nc duals when periodic, nc+1 otherwise (the half-duals at the walls);
+// a factor-1 direction has one dual per cell.
+__host__ __device__ inline int prolong_duals_dir(int nc, int fac, int per) {
+  return fac == 1 ? nc : (per ? nc : nc + 1);
+}
+__host__ __device__ inline int prolong_duals(const DevLevel& F, const DevLevel& C, const DevXfer& x) {
+  return prolong_duals_dir(C.nx, x.fx, F.px) * prolong_duals_dir(C.ny, x.fy, F.py) *
+         prolong_duals_dir(C.nz, x.fz, F.pz);
+}
+
 // Coarse tail: the levels of an instance at and below the first level with
 // at most kTailCells cells (and nz <= kTailNz for the z-line smoother) are
 // cycled inside one workgroup of 64*kTailWaves
@@ -78,16 +90,25 @@ constexpr int kTailNz = 32;      // z-line tail levels: whole columns in registe
 constexpr int kTailNz64 = 16;    // the same for FP64 (register budget)
 constexpr int kTailWaves = 8;
 constexpr int kTailMaxLevels = 16;
+// Steps of the tail's sub-V-cycle, precomputed by the host in cycle order:
+// code = type | level << 3 | parity << 7 | zero_guess << 8.
+enum { kStepZero = 0, kStepHalf = 1, kStepRestrict = 2, kStepCoarse = 3, kStepProlong = 4 };
+constexpr int kTailMaxSteps = 1024;
 struct DevTail {
   int nlev;                       // tail levels, the last is the coarsest
-  int zero_first;                 // the first tail level starts from u = 0
   int copy_u_in;                  // the first tail level's u is read (not zero)
   int zline;                      // smoother: 1 z-line, 0 point red-black
-  int lds_u[kTailMaxLevels];      // LDS offsets (elements of the level precision)
+  int nsteps;
+  int steps[kTailMaxSteps];
+  // LDS byte offsets (16-byte aligned) per tail level: u and f (level
+  // precision), the coefficient table (3*(nx+ny+nz) values of the level
+  // precision) and the cell widths (nx+ny+nz doubles; not for the coarsest)
+  int lds_u[kTailMaxLevels];
   int lds_f[kTailMaxLevels];
+  int lds_co[kTailMaxLevels];
+  int lds_vol[kTailMaxLevels];
   int lev[kTailMaxLevels];        // DevLevel index per tail level
   int xf[kTailMaxLevels];         // DevXfer index of tail level d -> d+1
-  int pre[kTailMaxLevels], post[kTailMaxLevels];
 };
 
 void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
@@ -99,6 +120,15 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
 // waves (segments of kZSeg cells) per column group; every item of one
 // launch must have nz <= segs*kZSeg.
 constexpr int kZSeg = 16;
+// rows of a segment whose right-hand-side loads are issued together
+#ifndef AMG_ZCHUNK32
+#define AMG_ZCHUNK32 16
+#endif
+#ifndef AMG_ZCHUNK64
+#define AMG_ZCHUNK64 8
+#endif
+constexpr int kZChunk32 = AMG_ZCHUNK32;
+constexpr int kZChunk64 = AMG_ZCHUNK64;
 constexpr int kZMaxSegs = 16;
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                   int zero_guess, int segs, hipStream_t s);
@@ -113,13 +143,17 @@ void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int pa
 // one workgroup per item (items index DevTail entries), lds_bytes of
 // dynamic LDS (the largest tail of the items); tail_lds_limit() is the most
 // a launch may request on this device.
-void launch_tail(bool fp64, const DevLevel* L, const DevXfer* X, const DevTail* T,
-                 const LaunchList& ll, size_t lds_bytes, hipStream_t s);
+// variant: the kernel's column length class (tail_nz_class) and whether
+// every tail transfer coarsens 2:1 in all three directions.
+int tail_nz_class(bool fp64, int nz_max);
+void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
+                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s);
 size_t tail_lds_limit();
 
 // ---- FP64 finest-level kernels (items index FP64 finest entries) -------------
-// Each thread handles kFinePer cells; each block writes nvals partial maxima
-// to partial[block*nvals + v] (blocks per item: cdiv(ncell, kBlock*kFinePer));
+// Each block writes nvals partial maxima to partial[block*nvals + v]; the
+// stencil kernels march in z (blocks per item: march_blocks), the change
+// kernel handles kFinePer cells per thread (cdiv(ncell, kBlock*kFinePer));
 // launch_finalize folds them per item into out[item*ostride + v].
 // residual_max:     {max|b - A u|, max|b|}
 // change_max:       {max|u - u2|, max|u|}        (u2 holds the previous iterate)
@@ -127,6 +161,13 @@ size_t tail_lds_limit();
 // correct_defect32: u2 = u + e (e = FP32 u of 'link'); r = b - A u2 -> FP32 f;
 //                   {max|r|, max|e|, max|u2|}
 constexpr int kFinePer = 8;
+// The FP64 finest-level stencil kernels (residual, defect, correct+defect)
+// march in z over slabs of kMarchK planes: blocks per item =
+// cdiv(2*hx*ny, kBlock) * cdiv(nz, kMarchK), one partial per block.
+constexpr int kMarchK = 16;
+inline long long march_blocks(int hx, int ny, int nz) {
+  return ((2LL * hx * ny + kBlock - 1) / kBlock) * ((nz + kMarchK - 1) / kMarchK);
+}
 void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
 void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
 void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);

@@ -4,12 +4,31 @@
 // step of several instances/levels (LaunchList in amg_internal.h), so a
 // single solve and a batch of uneven solves share the same code.  All
 // kernels are HBM-streaming stencils (about 0.5 flop/byte): no MFMA.
+//
+// Field pointers carry their address space: global (1) in the standalone
+// kernels, LDS (3) inside the coarse tail.  Generic (flat) accesses would
+// make every load wait for both the vector-memory and the LDS counters and
+// serialise the loads of a line solve; the element routines are therefore
+// templates over the pointer types and the kernels cast the level
+// descriptors' pointers before use.  Chequer indices are 32-bit.
 #include "amg_internal.h"
 
 #include <cmath>
 
 namespace amg {
 namespace {
+
+template <int AS, class T> using sp = __attribute__((address_space(AS))) T*;
+constexpr int kG = 1;   // global memory
+constexpr int kS = 3;   // LDS
+
+template <class T> __device__ __forceinline__ sp<kG, T> gp(T* p) { return (sp<kG, T>)p; }
+template <class T> __device__ __forceinline__ sp<kG, T> gfield(void* p) {
+  return (sp<kG, T>)static_cast<T*>(p);
+}
+template <class T> __device__ __forceinline__ sp<kG, const T> gfield(const void* p) {
+  return (sp<kG, const T>)static_cast<const T*>(p);
+}
 
 __device__ __forceinline__ int find_item(const int* prefix, int n, int b) {
   int lo = 0, hi = n - 1;
@@ -33,9 +52,13 @@ __device__ __forceinline__ int xcd_swizzle(int b, int n) {
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
 
-template <class T> __device__ __forceinline__ const T* coefs(const DevLevel& L);
-template <> __device__ __forceinline__ const double* coefs<double>(const DevLevel& L) { return L.co64; }
-template <> __device__ __forceinline__ const float* coefs<float>(const DevLevel& L) { return L.co32; }
+template <class T> __device__ __forceinline__ sp<kG, const T> coefs(const DevLevel& L);
+template <> __device__ __forceinline__ sp<kG, const double> coefs<double>(const DevLevel& L) {
+  return gp(L.co64);
+}
+template <> __device__ __forceinline__ sp<kG, const float> coefs<float>(const DevLevel& L) {
+  return gp(L.co32);
+}
 
 // NaN-propagating maximum (fmax ignores NaN).
 __device__ __forceinline__ double nmax(double a, double b) {
@@ -74,13 +97,13 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* __re
 }
 
 // Chequer-layout index of (m, j, k) in the half-row of colour c.
-__device__ __forceinline__ size_t cidx(int m, int j, int k, int c, int hx, int ny) {
-  return ((size_t)(k * ny + j) * 2 + c) * hx + m;
+__device__ __forceinline__ int cidx(int m, int j, int k, int c, int hx, int ny) {
+  return ((k * ny + j) * 2 + c) * hx + m;
 }
 
 // Seven chequer indices of cell (i,j,k): centre, x-, x+, y-, y+, z-, z+.
 struct Nb {
-  size_t c, xm, xp, ym, yp, zm, zp;
+  int c, xm, xp, ym, yp, zm, zp;
 };
 
 __device__ __forceinline__ Nb neighbours(const DevLevel& L, int i, int j, int k) {
@@ -92,7 +115,7 @@ __device__ __forceinline__ Nb neighbours(const DevLevel& L, int i, int j, int k)
   const int km = (k == 0) ? nz - 1 : k - 1, kp = (k == nz - 1) ? 0 : k + 1;
   Nb n;
   n.c = cidx(m, j, k, col, hx, ny);
-  const size_t ro = cidx(0, j, k, oth, hx, ny);
+  const int ro = cidx(0, j, k, oth, hx, ny);
   n.xm = ro + ml;
   n.xp = ro + mr;
   n.ym = cidx(m, jm, k, oth, hx, ny);
@@ -114,12 +137,12 @@ __device__ __forceinline__ void decode(const DevLevel& L, int e, int& i, int& j,
 }
 
 // Off-diagonal stencil sum, in a fixed order: x-, x+, y-, y+, z-, z+.
-template <class T>
-__device__ __forceinline__ T offsum(const T* __restrict__ co, const DevLevel& L, int i, int j, int k,
-                                    T uxm, T uxp, T uym, T uyp, T uzm, T uzp) {
+template <class T, class CP>
+__device__ __forceinline__ T offsum(CP co, const DevLevel& L, int i, int j, int k, T uxm, T uxp,
+                                    T uym, T uyp, T uzm, T uzp) {
   const int nx = L.nx, ny = L.ny;
-  const T* cy = co + 3 * nx;
-  const T* cz = cy + 3 * ny;
+  const auto cy = co + 3 * nx;
+  const auto cz = cy + 3 * ny;
   T s = co[i] * uxm + co[nx + i] * uxp;
   s += cy[j] * uym;
   s += cy[ny + j] * uyp;
@@ -128,11 +151,11 @@ __device__ __forceinline__ T offsum(const T* __restrict__ co, const DevLevel& L,
   return s;
 }
 
-template <class T>
-__device__ __forceinline__ T diag_at(const T* __restrict__ co, const DevLevel& L, int i, int j, int k) {
+template <class T, class CP>
+__device__ __forceinline__ T diag_at(CP co, const DevLevel& L, int i, int j, int k) {
   const int nx = L.nx, ny = L.ny;
-  const T* cy = co + 3 * nx;
-  const T* cz = cy + 3 * ny;
+  const auto cy = co + 3 * nx;
+  const auto cz = cy + 3 * ny;
   return (cz[2 * L.nz + k] + cy[2 * ny + j]) + co[2 * nx + i];
 }
 
@@ -141,15 +164,13 @@ __device__ __forceinline__ T diag_at(const T* __restrict__ co, const DevLevel& L
 // u = (f - sum w_n u_n)/diag on the cells with (i+j+k)&1 == parity.  Element
 // e enumerates the cells of the colour; in plane k the colour is column
 // colour (parity+k)&1, so consecutive e are one contiguous half-row.
-template <class T>
-__device__ __forceinline__ void rb_point_elem(const DevLevel& L, int e, int parity, int zero_guess) {
+template <class T, class UP, class FP, class CP>
+__device__ __forceinline__ void rb_point_elem(const DevLevel& L, UP u, FP f, CP co, int e,
+                                              int parity, int zero_guess) {
   const int hx = L.hx, ny = L.ny;
   const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
   const int c = (k + parity) & 1;
   const int i = 2 * m + ((c + j) & 1);
-  const T* __restrict__ co = coefs<T>(L);
-  T* __restrict__ u = static_cast<T*>(L.u);
-  const T* __restrict__ f = static_cast<const T*>(L.f);
   const Nb n = neighbours(L, i, j, k);
   const T d = diag_at<T>(co, L, i, j, k);
   T val;
@@ -169,31 +190,25 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e < L.hx * L.ny * L.nz) rb_point_elem<T>(L, e, parity, zero_guess);
+  if (e < L.hx * L.ny * L.nz)
+    rb_point_elem<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), e, parity,
+                     zero_guess);
 }
 
 // ---------------------------------------------------------------------------
-// z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-538; the
-// reference solves each line with Thomas, :541-595): on every z-column of
-// colour (i+j)&1 == parity solve exactly
-//     wzm_k u_{k-1} + (dz_k + dx_i + dy_j) u_k + wzp_k u_{k+1} = g_k,
-//     g = f - (x/y neighbour terms, other colour).
-// A periodic z wrap is lagged (old values), as the reference lags ghosts.
-//
-// Partitioned solve: the column is cut into segments of S cells; wave p of a
-// column group owns segment p of 64 columns (one column per lane, the
-// segment's RHS in registers).  Each segment T_p is solved locally for its
-// interface data -- the first/last entries of y = T_p^-1 g and of the two
-// spikes v = T_p^-1 (a e_1), w = T_p^-1 (c e_S), from one LU (top-down) and
-// one UL (bottom-up) sweep without storage -- then wave 0 solves the
-// 2x2-block reduced system of segment end values in LDS, and every segment
-// re-solves with its end couplings moved to the right-hand side.  All loads
-// are contiguous half-rows of the chequer layout; nothing but u is written.
-// One column group: 64 columns (lane = column) x P segments (wave p).  Every
-// wave of the workgroup must call it together: it synchronises the
-// workgroup when a column has more than one segment.  'lds' holds 6*P*64
-// values of T for this group; 'act' false makes the group idle.
-template <class T, int S>
+// z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-600, LJ):
+// the columns (i,j) of one colour ((i+j)&1 == parity) solve their
+// tridiagonal z-system with the other colour's current values on the
+// right-hand side.  A workgroup takes 64 columns (one per lane) and splits
+// each column into P segments of S cells (wave p owns segment p): every
+// segment eliminates its rows forward (LU) and backward (UL) to express its
+// end values through the unknown end values of its neighbours, the 2P
+// interface unknowns are solved by one wave in LDS, and every segment then
+// solves its rows with the interface values known.  The result equals a
+// Thomas solve of the whole column up to rounding.  Periodic z: the wrap
+// couplings are lagged (the other end's old value on the right-hand side),
+// in the order of the oracle's restatement.
+template <class T, int S, int C>
 __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int zero_guess, int col0,
                                             int p, int lane, int P, T* __restrict__ lds, bool act) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
@@ -206,59 +221,106 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
   int mr = m + r;     if (mr >= hx) mr = 0;
   const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
-  const T* __restrict__ co = coefs<T>(L);
-  const T* cy = co + 3 * nx;
-  const T* cz = cy + 3 * ny;
+  const sp<kG, const T> co = coefs<T>(L);
+  const sp<kG, const T> cy = co + 3 * nx;
+  const sp<kG, const T> cz = cy + 3 * ny;
   const T wxl = co[i], wxr = co[nx + i], wys = cy[j], wyn = cy[ny + j];
   const T dxy = co[2 * nx + i] + cy[2 * ny + j];
-  T* __restrict__ u = static_cast<T*>(L.u);
-  const T* __restrict__ f = static_cast<const T*>(L.f);
-  const size_t plane = (size_t)ny * 2 * hx;
-  const size_t own = cidx(m, j, 0, c, hx, ny);
-  const size_t xo = cidx(0, j, 0, oth, hx, ny);
-  const size_t so = cidx(m, jm, 0, oth, hx, ny);
-  const size_t no = cidx(m, jp, 0, oth, hx, ny);
+  const sp<kG, T> u = gfield<T>(L.u);
+  const sp<kG, const T> f = gfield<const T>((const void*)L.f);
+  const int plane = ny * 2 * hx;
+  const int own = cidx(m, j, 0, c, hx, ny);
+  const int xo = cidx(0, j, 0, oth, hx, ny);
+  const int so = cidx(m, jm, 0, oth, hx, ny);
+  const int no = cidx(m, jp, 0, oth, hx, ny);
   const bool pz = L.pz != 0;
   const int Pi = (nz + S - 1) / S;            // segments of this item (<= blockDim.y)
   const int ks = p * S;
   const int len = max(0, min(S, nz - ks));    // 0 for spare waves of a shorter item
-  // Rows of the segment, clamped so that every load below is in bounds and
-  // the unrolled loops stay branch-free (all loads in flight at once).
-  // A: sub-diagonal, E: super-diagonal, D: diagonal; the lagged periodic
-  // wrap couplings are zero in the matrix.
-  T A[S], E[S], D[S], g[S];
+  // A (sub-diagonal), E (super-diagonal) and the z part of the diagonal
+  // depend on k only and p is wave-uniform, so they are scalar values
+  // (uniform loads issued with the rest, read without memory latency in the
+  // serial elimination chains); the lane's x/y diagonal share dxy is added
+  // at use.  The lagged periodic wrap couplings are zero in the matrix.
+  // All loads -- these rows, the lagged wrap values and the right-hand-side
+  // data -- are unconditional (every address is in bounds, uses are
+  // selected) and issued before the first wait: a sweep pays one memory
+  // round trip.
+  // FP64 rows do not fit in scalar registers: they are staged in LDS by
+  // lanes 0..S-1 and read back as broadcasts (LDS_COEF).
+  constexpr bool LDS_COEF = sizeof(T) == 8;
+  constexpr int SR = LDS_COEF ? 1 : S;
+  T A[SR], E[SR], Dz[SR];
+  T* const cA = lds + 6 * P * 64 + p * 3 * S;
+  T* const cE = cA + S;
+  T* const cD = cE + S;
+  T za = T(0), ze = T(0), zd = T(0);
+  if constexpr (LDS_COEF) {
+    const int kq = min(ks + (lane & (S - 1)), nz - 1);
+    const T a0 = cz[kq], e0 = cz[nz + kq];
+    za = (pz && kq == 0) ? T(0) : a0;
+    ze = (pz && kq == nz - 1) ? T(0) : e0;
+    zd = cz[2 * nz + kq];
+  } else {
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    const int k = min(ks + q, nz - 1);
-    A[q] = (pz && k == 0) ? T(0) : cz[k];
-    E[q] = (pz && k == nz - 1) ? T(0) : cz[nz + k];
-    D[q] = cz[2 * nz + k] + dxy;
+    for (int q = 0; q < S; ++q) {
+      const int k = min(ks + q, nz - 1);
+      const T a0 = cz[k], e0 = cz[nz + k];
+      A[q] = (pz && k == 0) ? T(0) : a0;
+      E[q] = (pz && k == nz - 1) ? T(0) : e0;
+      Dz[q] = cz[2 * nz + k];
+    }
   }
+  const T cz_first = cz[0], cz_last = cz[2 * nz - 1];
   // zero_guess bit 0: the other colour is zero (first half-sweep from u = 0);
   // bit 1: this colour's old values are zero (no lagged wrap coupling)
   const bool nbr = !(zero_guess & 1);
   const bool lag = pz && !(zero_guess & 2);
-  const T lagb = (lag && p == 0) ? cz[0] * u[(size_t)(nz - 1) * plane + own] : T(0);
-  const T lagt = (lag && ks + len == nz && len > 0) ? cz[2 * nz - 1] * u[own] : T(0);
+  const T ubot = u[(nz - 1) * plane + own], utop = u[own];
+  const T lagb = (lag && p == 0) ? cz_first * ubot : T(0);
+  const T lagt = (lag && ks + len == nz && len > 0) ? cz_last * utop : T(0);
+  T g[S];
 #pragma unroll
-  for (int q = 0; q < S; ++q) {
-    const int k = min(ks + q, nz - 1);
-    const size_t kb = (size_t)k * plane;
-    T v = f[kb + own];
-    if (nbr) {
-      T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
-      s += wys * u[kb + so];
-      s += wyn * u[kb + no];
-      v = v - s;
+  for (int q0 = 0; q0 < S; q0 += C) {
+    // keep this chunk's loads behind the previous chunk's arithmetic
+    if (q0 > 0) __builtin_amdgcn_sched_barrier(0);
+    T fv[C], xl[C], xr[C], ys[C], yn[C];
+#pragma unroll
+    for (int qq = 0; qq < C; ++qq) {
+      const int kb = min(ks + q0 + qq, nz - 1) * plane;   // rows clamped: in bounds
+      fv[qq] = f[kb + own];
+      xl[qq] = u[kb + xo + ml];
+      xr[qq] = u[kb + xo + mr];
+      ys[qq] = u[kb + so];
+      yn[qq] = u[kb + no];
     }
-    v -= (ks + q == 0) ? lagb : T(0);
-    v -= (ks + q == nz - 1) ? lagt : T(0);
-    g[q] = (q < len) ? v : T(0);
+    if (LDS_COEF && q0 == 0) {
+      if (lane < S) {
+        cA[lane] = za;
+        cE[lane] = ze;
+        cD[lane] = zd;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int qq = 0; qq < C; ++qq) {
+      const int q = q0 + qq;
+      T sn = wxl * xl[qq] + wxr * xr[qq];
+      sn += wys * ys[qq];
+      sn += wyn * yn[qq];
+      T v = nbr ? fv[qq] - sn : fv[qq];
+      v -= (ks + q == 0) ? lagb : T(0);
+      v -= (ks + q == nz - 1) ? lagt : T(0);
+      g[q] = (q < len) ? v : T(0);
+    }
   }
-  const T a_first = A[0];                                  // coupling to u_{ks-1}
+#define A(q) (LDS_COEF ? cA[q] : A[LDS_COEF ? 0 : (q)])
+#define E(q) (LDS_COEF ? cE[q] : E[LDS_COEF ? 0 : (q)])
+#define D(q) ((LDS_COEF ? cD[q] : Dz[LDS_COEF ? 0 : (q)]) + dxy)
+  const T a_first = A(0);                                  // coupling to u_{ks-1}
   T c_last = T(0);                                         // coupling to u_{ks+len}
 #pragma unroll
-  for (int q = 0; q < S; ++q) c_last = (q == len - 1) ? E[q] : c_last;
+  for (int q = 0; q < S; ++q) c_last = (q == len - 1) ? E(q) : c_last;
   // phase 1: interface data of segment p
   T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
   if (len > 0) {
@@ -267,14 +329,14 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       if (q < len) {
-        const T a = (q == 0) ? T(0) : A[q];
-        const T rd = rcp(D[q] - a * cp);
+        const T a = (q == 0) ? T(0) : A(q);
+        const T rd = rcp(D(q) - a * cp);
         yf = (g[q] - a * yf) * rd;
         vf = (q == 0) ? a_first * rd : -a * vf * rd;
         if (q == len - 1) {
           wl = c_last * rd;
         } else {
-          cp = E[q] * rd;
+          cp = E(q) * rd;
           prod *= -cp;
         }
       }
@@ -288,11 +350,11 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     for (int q = S - 1; q >= 0; --q) {
       if (q < len) {
         if (q < len - 1) {
-          const T ri2 = rcp(D[q] - E[q] * A[q + 1] * ri);
-          h = (g[q] - E[q] * h) * ri2;
+          const T ri2 = rcp(D(q) - E(q) * A(q + 1) * ri);
+          h = (g[q] - E(q) * h) * ri2;
           ri = ri2;
         } else {
-          ri = rcp(D[q]);
+          ri = rcp(D(q));
           h = g[q] * ri;
         }
       }
@@ -319,11 +381,12 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
       T beta = T(0), gamma = T(0);
       for (int q = 0; q < Pi; ++q) {
         const int s2 = q * 64 + lane;
-        const T dn = T(1) / (T(1) - V0[s2] * gamma);
-        const T tau = (Y0[s2] - V0[s2] * beta) * dn;
-        const T sigma = W0[s2] * dn;
-        const T nb = YL[s2] - VL[s2] * beta + VL[s2] * gamma * tau;
-        const T ng = WL[s2] + VL[s2] * gamma * sigma;
+        const T v0 = V0[s2], y0q = Y0[s2], w0 = W0[s2], vl = VL[s2], yl = YL[s2], wl = WL[s2];
+        const T dn = rcp(T(1) - v0 * gamma);
+        const T tau = (y0q - v0 * beta) * dn;
+        const T sigma = w0 * dn;
+        const T nb = yl - vl * beta + vl * gamma * tau;
+        const T ng = wl + vl * gamma * sigma;
         V0[s2] = tau; W0[s2] = sigma; VL[s2] = nb; WL[s2] = ng;
         beta = nb; gamma = ng;
       }
@@ -353,10 +416,10 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
         T rhs = g[q];
         if (q == 0) rhs -= a_first * bprev;
         if (q == len - 1) rhs -= c_last * tnext;
-        const T a = (q == 0) ? T(0) : A[q];
-        const T rd = rcp(D[q] - a * cp);
+        const T a = (q == 0) ? T(0) : A(q);
+        const T rd = rcp(D(q) - a * cp);
         yf = (rhs - a * yf) * rd;
-        cp = (q == len - 1) ? T(0) : E[q] * rd;
+        cp = (q == len - 1) ? T(0) : E(q) * rd;
         g[q] = yf;
         cpa[q] = cp;
       }
@@ -366,20 +429,25 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     for (int q = S - 1; q >= 0; --q) {
       if (q < len) {
         x = g[q] - cpa[q] * x;
-        if (live) u[(size_t)(ks + q) * plane + own] = x;
+        if (live) u[(ks + q) * plane + own] = x;
       }
     }
   }
+#undef A
+#undef E
+#undef D
 }
 
-template <class T, int S>
+template <class T, int S, int C>
 __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                 int parity, int zero_guess) {
   extern __shared__ unsigned char lds_raw[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  zline_group<T, S>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, threadIdx.y,
+  // threadIdx.y is constant across a wave (blockDim.x == 64): say so to the compiler
+  const int seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  zline_group<T, S, C>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
                     threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
 }
 
@@ -387,11 +455,12 @@ template <class T> struct TailNz;
 template <> struct TailNz<float> { static constexpr int value = kTailNz; };
 template <> struct TailNz<double> { static constexpr int value = kTailNz64; };
 
-// Short columns (nz <= TailNz) inside the coarse tail: one thread per
-// column, the whole column in registers, plain Thomas (the reference's line
-// solve, mgMK/mg_relax.F90:541-595); column index col = j*hx + m.
-template <class T, int NZ>
-__device__ __forceinline__ void zline_column(const DevLevel& L, int parity, int zero_guess, int col) {
+// Short columns (nz <= NZ) inside the coarse tail: one thread per column,
+// the whole column in registers, plain Thomas (the reference's line solve,
+// mgMK/mg_relax.F90:541-595); column index col = j*hx + m.
+template <class T, int NZ, class UP, class FP, class CP>
+__device__ __forceinline__ void zline_column(const DevLevel& L, UP u, FP f, CP co, int parity,
+                                             int zero_guess, int col) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
   const int m = col % hx, j = col / hx;
   const int c = parity, oth = parity ^ 1;
@@ -399,37 +468,35 @@ __device__ __forceinline__ void zline_column(const DevLevel& L, int parity, int 
   int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
   int mr = m + r;     if (mr >= hx) mr = 0;
   const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
-  const T* __restrict__ co = coefs<T>(L);
-  const T* cy = co + 3 * nx;
-  const T* cz = cy + 3 * ny;
+  const auto cy = co + 3 * nx;
+  const auto cz = cy + 3 * ny;
   const T wxl = co[i], wxr = co[nx + i], wys = cy[j], wyn = cy[ny + j];
   const T dxy = co[2 * nx + i] + cy[2 * ny + j];
-  T* __restrict__ u = static_cast<T*>(L.u);
-  const T* __restrict__ f = static_cast<const T*>(L.f);
-  const size_t plane = (size_t)ny * 2 * hx;
-  const size_t own = cidx(m, j, 0, c, hx, ny);
-  const size_t xo = cidx(0, j, 0, oth, hx, ny);
-  const size_t so = cidx(m, jm, 0, oth, hx, ny);
-  const size_t no = cidx(m, jp, 0, oth, hx, ny);
+  const int plane = ny * 2 * hx;
+  const int own = cidx(m, j, 0, c, hx, ny);
+  const int xo = cidx(0, j, 0, oth, hx, ny);
+  const int so = cidx(m, jm, 0, oth, hx, ny);
+  const int no = cidx(m, jp, 0, oth, hx, ny);
   const bool pz = L.pz != 0;
   const bool lag = pz && !(zero_guess & 2);
-  const T lagb = lag ? cz[0] * u[(size_t)(nz - 1) * plane + own] : T(0);
+  const T lagb = lag ? cz[0] * u[(nz - 1) * plane + own] : T(0);
   const T lagt = lag ? cz[2 * nz - 1] * u[own] : T(0);
   T g[NZ], cpa[NZ];
 #pragma unroll
   for (int q = 0; q < NZ; ++q) {
-    const int k = min(q, nz - 1);
-    const size_t kb = (size_t)k * plane;
-    T v = f[kb + own];
-    if (!(zero_guess & 1)) {
-      T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
-      s += wys * u[kb + so];
-      s += wyn * u[kb + no];
-      v = v - s;
+    if (q < nz) {
+      const int kb = q * plane;
+      T v = f[kb + own];
+      if (!(zero_guess & 1)) {
+        T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
+        s += wys * u[kb + so];
+        s += wyn * u[kb + no];
+        v = v - s;
+      }
+      v -= (q == 0) ? lagb : T(0);
+      v -= (q == nz - 1) ? lagt : T(0);
+      g[q] = v;
     }
-    v -= (q == 0) ? lagb : T(0);
-    v -= (q == nz - 1) ? lagt : T(0);
-    g[q] = v;
   }
   T cp = T(0), yf = T(0);
 #pragma unroll
@@ -449,9 +516,20 @@ __device__ __forceinline__ void zline_column(const DevLevel& L, int parity, int 
   for (int q = NZ - 1; q >= 0; --q) {
     if (q < nz) {
       x = g[q] - cpa[q] * x;
-      u[(size_t)q * plane + own] = x;
+      u[q * plane + own] = x;
     }
   }
+}
+
+// Dispatch on the column length: the smallest unrolled length that holds nz.
+template <class T, class UP, class FP, class CP>
+__device__ __forceinline__ void zline_column_any(const DevLevel& L, UP u, FP f, CP co, int parity,
+                                                 int zero_guess, int col) {
+  const int nz = L.nz;
+  if (nz <= 4) zline_column<T, 4>(L, u, f, co, parity, zero_guess, col);
+  else if (nz <= 8) zline_column<T, 8>(L, u, f, co, parity, zero_guess, col);
+  else if (nz <= 16 || TailNz<T>::value <= 16) zline_column<T, 16>(L, u, f, co, parity, zero_guess, col);
+  else zline_column<T, TailNz<T>::value>(L, u, f, co, parity, zero_guess, col);
 }
 
 template <class T>
@@ -461,30 +539,34 @@ __global__ __launch_bounds__(kBlock) void k_zline_col(const DevLevel* __restrict
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e < L.hx * L.ny) zline_column<T, TailNz<T>::value>(L, parity, zero_guess, e);
+  if (e < L.hx * L.ny)
+    zline_column_any<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), parity,
+                        zero_guess, e);
 }
 
 // ---------------------------------------------------------------------------
 // Fused residual + volume-weighted restriction (mgMK/mg_restr.F90:124-164):
 // element e is a coarse cell in natural (I fastest) order, so the fine reads
 // of a wave are contiguous half-rows; the fine residual is never stored.
-template <class T>
-__device__ __forceinline__ void restrict_elem(const DevLevel& F, const DevLevel& C, const DevXfer& x,
-                                              int e) {
+// FX, FY, FZ: the coarsening factors (1 or 2), compile-time so that the
+// children loop unrolls and its loads are issued together.
+template <class T, int FX, int FY, int FZ, class UP, class FP, class CP, class VP, class CFP>
+__device__ __forceinline__ void restrict_elem(const DevLevel& F, UP u, FP f, CP co, VP vol,
+                                              const DevLevel& C, CFP fc, int e) {
   const int I = e % C.nx, JK = e / C.nx, J = JK % C.ny, K = JK / C.ny;
-  const T* __restrict__ co = coefs<T>(F);
-  const T* __restrict__ u = static_cast<const T*>(F.u);
-  const T* __restrict__ f = static_cast<const T*>(F.f);
-  const double* vx = F.vol;
-  const double* vy = vx + F.nx;
-  const double* vz = vy + F.ny;
+  const auto vx = vol;
+  const auto vy = vx + F.nx;
+  const auto vz = vy + F.ny;
   T acc = T(0), vs = T(0);
-  for (int cz = 0; cz < x.fz; ++cz) {
-    const int k = K * x.fz + cz;
-    for (int cy = 0; cy < x.fy; ++cy) {
-      const int j = J * x.fy + cy;
-      for (int cx = 0; cx < x.fx; ++cx) {
-        const int i = I * x.fx + cx;
+#pragma unroll
+  for (int cz = 0; cz < FZ; ++cz) {
+    const int k = K * FZ + cz;
+#pragma unroll
+    for (int cy = 0; cy < FY; ++cy) {
+      const int j = J * FY + cy;
+#pragma unroll
+      for (int cx = 0; cx < FX; ++cx) {
+        const int i = I * FX + cx;
         const Nb n = neighbours(F, i, j, k);
         const T s = offsum<T>(co, F, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
         const T rr = f[n.c] - (s + diag_at<T>(co, F, i, j, k) * u[n.c]);
@@ -494,8 +576,23 @@ __device__ __forceinline__ void restrict_elem(const DevLevel& F, const DevLevel&
       }
     }
   }
-  T* __restrict__ fc = static_cast<T*>(C.f);
   fc[cidx(I >> 1, J, K, (I + J) & 1, C.hx, C.ny)] = acc / vs;
+}
+
+template <class T, class UP, class FP, class CP, class VP, class CFP>
+__device__ __forceinline__ void restrict_any(const DevLevel& F, UP u, FP f, CP co, VP vol,
+                                             const DevLevel& C, CFP fc, const DevXfer& x, int e) {
+  const int code = (x.fx - 1) * 4 + (x.fy - 1) * 2 + (x.fz - 1);
+  switch (code) {
+    case 7: restrict_elem<T, 2, 2, 2>(F, u, f, co, vol, C, fc, e); break;
+    case 6: restrict_elem<T, 2, 2, 1>(F, u, f, co, vol, C, fc, e); break;
+    case 5: restrict_elem<T, 2, 1, 2>(F, u, f, co, vol, C, fc, e); break;
+    case 4: restrict_elem<T, 2, 1, 1>(F, u, f, co, vol, C, fc, e); break;
+    case 3: restrict_elem<T, 1, 2, 2>(F, u, f, co, vol, C, fc, e); break;
+    case 2: restrict_elem<T, 1, 2, 1>(F, u, f, co, vol, C, fc, e); break;
+    case 1: restrict_elem<T, 1, 1, 2>(F, u, f, co, vol, C, fc, e); break;
+    default: restrict_elem<T, 1, 1, 1>(F, u, f, co, vol, C, fc, e); break;
+  }
 }
 
 template <class T>
@@ -505,57 +602,153 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevXfer x = X[ll.items[t]];
   const DevLevel& C = Lt[x.coarse];
+  const DevLevel& F = Lt[x.fine];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e < C.ncell) restrict_elem<T>(Lt[x.fine], C, x, e);
-}
-
-// 1-D prolongation weights of fine index i: coarse (I0, w0), (I1, w1).
-__device__ __forceinline__ void pw(int i, int nc, int fac, int per, int bc0, int bc1, int& I0,
-                                   int& I1, double& w0, double& w1) {
-  if (fac == 1) { I0 = I1 = i; w0 = 1.0; w1 = 0.0; return; }
-  I0 = i >> 1;
-  int far = (i & 1) ? I0 + 1 : I0 - 1;
-  w0 = 0.75; w1 = 0.25;
-  if (far < 0 || far >= nc) {
-    const int wall = (far < 0) ? bc0 : bc1;
-    if (per) far = (far + nc) % nc;
-    else if (wall == 2) { far = I0; w1 = 0.0; }
-    else far = I0;
-  }
-  I1 = far;
+  if (e < C.ncell)
+    restrict_any<T>(F, gfield<const T>((const void*)F.u), gfield<const T>((const void*)F.f),
+                    coefs<T>(F), gp(F.vol), C, gfield<T>(C.f), x, e);
 }
 
 // ---------------------------------------------------------------------------
 // Trilinear prolongation-add (mgMK/mg_cor.F90:102-159): fine u += sum of
 // 27,9,9,3,9,3,3,1 /64 of the eight nearest coarse cells; coarse ghosts are
-// periodic wraps, Neumann copies or Dirichlet zeros.  Element e is a fine
-// cell in chequer order (contiguous read-modify-write of the fine field).
-template <class T>
-__device__ __forceinline__ void prolong_elem(const DevLevel& F, const DevLevel& C, const DevXfer& x,
-                                             int e) {
-  int i, j, k;
-  decode(F, e, i, j, k);
-  int I[2], J[2], K[2];
-  double wx[2], wy[2], wz[2];
-  pw(i, C.nx, x.fx, F.px, F.bcx0, F.bcx1, I[0], I[1], wx[0], wx[1]);
-  pw(j, C.ny, x.fy, F.py, F.bcy0, F.bcy1, J[0], J[1], wy[0], wy[1]);
-  pw(k, C.nz, x.fz, F.pz, F.bcz0, F.bcz1, K[0], K[1], wz[0], wz[1]);
-  const T* __restrict__ uc = static_cast<const T*>(C.u);
-  T acc = T(0);
-#pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    T sy = T(0);
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const T v0 = uc[cidx(I[0] >> 1, J[q], K[a], (I[0] + J[q]) & 1, C.hx, C.ny)];
-      const T v1 = uc[cidx(I[1] >> 1, J[q], K[a], (I[1] + J[q]) & 1, C.hx, C.ny)];
-      const T sx = T(wx[0]) * v0 + T(wx[1]) * v1;
-      sy += T(wy[q]) * sx;
-    }
-    acc += T(wz[a]) * sy;
+// periodic wraps, Neumann copies or Dirichlet zeros.
+//
+// One thread per dual cell (prolong_duals): the fine cells between coarse
+// centres I and I+1 in each direction share the same eight coarse values,
+// which are loaded once.  Per factor-2 direction the dual d covers fine
+// cells 2I+1 (weights 3/4 on I, 1/4 on I+1) and 2I+2 (3/4 on I+1, 1/4 on I)
+// with I = d (periodic, I+1 and 2I+2 wrapped) or I = d-1 (walls: the half
+// duals d = 0 and d = nc hold one fine cell; the missing coarse value is
+// the ghost -- the neighbour's value for Neumann, zero for Dirichlet).  A
+// factor-1 direction maps cell to cell with weight 1.  Each fine value is
+// summed nearest-first: x, then y, then z.
+struct PDir {
+  int fi0, fi1;     // fine indices (fine cell 0 is nearest to coarse centre 0)
+  bool fv0, fv1;    // fine cell exists
+  int ci0, ci1;     // coarse indices
+  bool cz0, cz1;    // coarse value is a Dirichlet zero ghost
+  __device__ int fi(int h) const { return h ? fi1 : fi0; }
+  __device__ bool fv(int h) const { return h ? fv1 : fv0; }
+  __device__ int ci(int h) const { return h ? ci1 : ci0; }
+  __device__ bool cz(int h) const { return h ? cz1 : cz0; }
+};
+
+template <int F>
+__device__ __forceinline__ PDir pdir(int d, int nc, int per, int bc0, int bc1) {
+  PDir r;
+  if (F == 1) {
+    r.fi0 = r.fi1 = d;
+    r.fv0 = true; r.fv1 = false;
+    r.ci0 = r.ci1 = d;
+    r.cz0 = r.cz1 = false;
+    return r;
   }
-  T* __restrict__ u = static_cast<T*>(F.u);
-  u[e] += acc;
+  const int nf = 2 * nc;
+  if (per) {
+    const int I = d;
+    r.fi0 = 2 * I + 1;
+    r.fi1 = (2 * I + 2 == nf) ? 0 : 2 * I + 2;
+    r.fv0 = r.fv1 = true;
+    r.ci0 = I;
+    r.ci1 = (d + 1 == nc) ? 0 : d + 1;
+    r.cz0 = r.cz1 = false;
+  } else {
+    const int I = d - 1;
+    r.fi0 = 2 * I + 1;
+    r.fi1 = 2 * I + 2;
+    r.fv0 = I >= 0;
+    r.fv1 = I + 1 < nc;
+    // a missing coarse centre takes the ghost rule of its wall
+    r.ci0 = (I >= 0) ? I : 0;
+    r.ci1 = (I + 1 < nc) ? I + 1 : nc - 1;
+    r.cz0 = (I < 0) && bc0 == 2;
+    r.cz1 = (I + 1 >= nc) && bc1 == 2;
+  }
+  return r;
+}
+
+template <class T, int FX, int FY, int FZ, class UP, class CUP>
+__device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevLevel& C, CUP uc,
+                                             int e) {
+  constexpr int NX = FX == 2 ? 2 : 1, NY = FY == 2 ? 2 : 1, NZ = FZ == 2 ? 2 : 1;
+  const int ndx = prolong_duals_dir(C.nx, FX, F.px), ndy = prolong_duals_dir(C.ny, FY, F.py);
+  const int dx = e % ndx, dyz = e / ndx, dy = dyz % ndy, dz = dyz / ndy;
+  const PDir X = pdir<FX>(dx, C.nx, F.px, F.bcx0, F.bcx1);
+  const PDir Y = pdir<FY>(dy, C.ny, F.py, F.bcy0, F.bcy1);
+  const PDir Z = pdir<FZ>(dz, C.nz, F.pz, F.bcz0, F.bcz1);
+  // every load first: the coarse values v[z][y][x] (index 0: the near
+  // centre of fine cell 0) and the fine cells' old values (a missing cell's
+  // index is clamped to its partner; it is not stored)
+  T v[NZ][NY][NX], fold[NZ][NY][NX];
+  int fid[NZ][NY][NX];
+#pragma unroll
+  for (int a = 0; a < NZ; ++a)
+#pragma unroll
+    for (int q = 0; q < NY; ++q)
+#pragma unroll
+      for (int h = 0; h < NX; ++h) {
+        const int I = X.ci(h), J = Y.ci(q), K = Z.ci(a);
+        // a Dirichlet ghost contributes zero: a 0/1 factor, not a branch,
+        // so that all loads stay in one batch (the same 0*value as a zero
+        // interpolation weight)
+        const T c = uc[cidx(I >> 1, J, K, (I + J) & 1, C.hx, C.ny)];
+        v[a][q][h] = c * ((X.cz(h) || Y.cz(q) || Z.cz(a)) ? T(0) : T(1));
+        const int i = X.fv(h) ? X.fi(h) : X.fi(h ^ (NX - 1));
+        const int j = Y.fv(q) ? Y.fi(q) : Y.fi(q ^ (NY - 1));
+        const int k = Z.fv(a) ? Z.fi(a) : Z.fi(a ^ (NZ - 1));
+        fid[a][q][h] = cidx(i >> 1, j, k, (i + j) & 1, F.hx, F.ny);
+        fold[a][q][h] = u[fid[a][q][h]];
+      }
+  // keep the old values' loads in the batch above (an empty register
+  // constraint: otherwise a load is sunk into its predicated store)
+#pragma unroll
+  for (int a = 0; a < NZ; ++a)
+#pragma unroll
+    for (int q = 0; q < NY; ++q)
+#pragma unroll
+      for (int h = 0; h < NX; ++h) asm volatile("" : "+v"(fold[a][q][h]));
+  // separable interpolation, nearest first: along x, then y, then z
+  T ix[NZ][NY][NX], iy[NZ][NY][NX];
+#pragma unroll
+  for (int a = 0; a < NZ; ++a)
+#pragma unroll
+    for (int q = 0; q < NY; ++q)
+#pragma unroll
+      for (int h = 0; h < NX; ++h)
+        ix[a][q][h] = (NX == 2) ? T(0.75) * v[a][q][h] + T(0.25) * v[a][q][h ^ (NX - 1)] : v[a][q][h];
+#pragma unroll
+  for (int a = 0; a < NZ; ++a)
+#pragma unroll
+    for (int q = 0; q < NY; ++q)
+#pragma unroll
+      for (int h = 0; h < NX; ++h)
+        iy[a][q][h] = (NY == 2) ? T(0.75) * ix[a][q][h] + T(0.25) * ix[a][q ^ (NY - 1)][h] : ix[a][q][h];
+#pragma unroll
+  for (int a = 0; a < NZ; ++a)
+#pragma unroll
+    for (int q = 0; q < NY; ++q)
+#pragma unroll
+      for (int h = 0; h < NX; ++h) {
+        const T add = (NZ == 2) ? T(0.75) * iy[a][q][h] + T(0.25) * iy[a ^ (NZ - 1)][q][h] : iy[a][q][h];
+        if (X.fv(h) && Y.fv(q) && Z.fv(a)) u[fid[a][q][h]] = fold[a][q][h] + add;
+      }
+}
+
+template <class T, class UP, class CUP>
+__device__ __forceinline__ void prolong_any(const DevLevel& F, UP u, const DevLevel& C, CUP uc,
+                                            const DevXfer& x, int e) {
+  const int code = (x.fx - 1) * 4 + (x.fy - 1) * 2 + (x.fz - 1);
+  switch (code) {
+    case 7: prolong_dual<T, 2, 2, 2>(F, u, C, uc, e); break;
+    case 6: prolong_dual<T, 2, 2, 1>(F, u, C, uc, e); break;
+    case 5: prolong_dual<T, 2, 1, 2>(F, u, C, uc, e); break;
+    case 4: prolong_dual<T, 2, 1, 1>(F, u, C, uc, e); break;
+    case 3: prolong_dual<T, 1, 2, 2>(F, u, C, uc, e); break;
+    case 2: prolong_dual<T, 1, 2, 1>(F, u, C, uc, e); break;
+    case 1: prolong_dual<T, 1, 1, 2>(F, u, C, uc, e); break;
+    default: prolong_dual<T, 1, 1, 1>(F, u, C, uc, e); break;
+  }
 }
 
 template <class T>
@@ -565,29 +758,29 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevXfer x = X[ll.items[t]];
   const DevLevel& F = Lt[x.fine];
+  const DevLevel& C = Lt[x.coarse];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e < F.ncell) prolong_elem<T>(F, Lt[x.coarse], x, e);
+  if (e < prolong_duals(F, C, x))
+    prolong_any<T>(F, gfield<T>(F.u), C, gfield<const T>((const void*)C.u), x, e);
 }
 
 // ---------------------------------------------------------------------------
 // Coarsest level: u = Minv f with a precomputed FP64 (pseudo-)inverse
-// (replaces the ipre+ipost sweeps of mgMK/mg_cycle.F90:117-148).  Minv is
-// stored transposed so the reads Minv^T[s][r] of consecutive rows r coalesce.
-template <class T>
-__device__ __forceinline__ void coarse_row(const DevLevel& L, int r) {
+// stored transposed (row r of the product reads column r: coalesced), FP64
+// accumulation in eight independent partial sums.
+template <class T, class UP, class FP>
+__device__ __forceinline__ void coarse_row(const DevLevel& L, UP u, FP f, int r) {
   const int n = L.ncell;
-  const T* __restrict__ f = static_cast<const T*>(L.f);
-  const double* __restrict__ M = L.minvT + r;
-  // 8 independent partial sums so that the loads of a row are in flight together
+  const sp<kG, const double> M = gp(L.minvT) + r;
   double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int s = 0;
   for (; s + 8 <= n; s += 8) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] += M[(size_t)(s + q) * n] * (double)f[s + q];
+    for (int q = 0; q < 8; ++q) acc[q] += M[(s + q) * n] * (double)f[s + q];
   }
-  for (; s < n; ++s) acc[0] += M[(size_t)s * n] * (double)f[s];
+  for (; s < n; ++s) acc[0] += M[s * n] * (double)f[s];
   const double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-  static_cast<T*>(L.u)[r] = (T)sum;
+  u[r] = (T)sum;
 }
 
 template <class T>
@@ -596,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int r = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (r < L.ncell) coarse_row<T>(L, r);
+  if (r < L.ncell) coarse_row<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), r);
 }
 
 // ---------------------------------------------------------------------------
@@ -605,11 +798,11 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
 // prolongation and post-smoothing back up -- the order of
 // mgMK/mg_cycle.F90) for one instance in one workgroup of 64*kTailWaves
 // threads, with workgroup barriers between the steps instead of dependent
-// launches.  The tail levels' u and f live in LDS (the first level's f is
-// copied in, its u copied out); the element routines are the same as in the
-// standalone kernels, reached through level descriptors whose field
-// pointers point into LDS.  One workgroup per item (DevTail).
-template <class T>
+// launches.  The tail levels' u, f, coefficient tables and cell widths live
+// in LDS (the first level's f, and u when it is an input, are copied in,
+// its u copied out); the element routines are the standalone kernels' ones
+// instantiated with LDS pointers.  One workgroup per item (DevTail).
+template <class T, int NZ, bool ALL222>
 __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
                                                           const DevXfer* __restrict__ X,
                                                           const DevTail* __restrict__ Tt,
@@ -619,66 +812,76 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
   const DevTail& tl = Tt[ll.items[blockIdx.x]];
   const int tid = threadIdx.y * 64 + threadIdx.x, nthr = 64 * blockDim.y;
   const int nl = tl.nlev;
-  T* buf = reinterpret_cast<T*>(lds_raw);
-  if (tid < nl) {
-    DevLevel d = Lt[tl.lev[tid]];
-    d.u = buf + tl.lds_u[tid];
-    d.f = buf + tl.lds_f[tid];
-    Ls[tid] = d;
+  const sp<kS, unsigned char> base = (sp<kS, unsigned char>)lds_raw;
+  auto U = [&](int d) { return (sp<kS, T>)(base + tl.lds_u[d]); };
+  auto Fv = [&](int d) { return (sp<kS, T>)(base + tl.lds_f[d]); };
+  auto CO = [&](int d) { return (sp<kS, const T>)(base + tl.lds_co[d]); };
+  auto VOL = [&](int d) { return (sp<kS, const double>)(base + tl.lds_vol[d]); };
+  if (tid < nl) Ls[tid] = Lt[tl.lev[tid]];
+  // stage the tables of every tail level and the first level's fields
+  for (int d = 0; d < nl; ++d) {
+    const DevLevel& G = Lt[tl.lev[d]];
+    const int nco = 3 * (G.nx + G.ny + G.nz), nv = G.nx + G.ny + G.nz;
+    const sp<kG, const T> gco = coefs<T>(G);
+    const sp<kS, T> lco = (sp<kS, T>)(base + tl.lds_co[d]);
+    for (int e = tid; e < nco; e += nthr) lco[e] = gco[e];
+    if (d + 1 < nl) {
+      const sp<kG, const double> gv = gp(G.vol);
+      const sp<kS, double> lv = (sp<kS, double>)(base + tl.lds_vol[d]);
+      for (int e = tid; e < nv; e += nthr) lv[e] = gv[e];
+    }
   }
   const DevLevel& G0 = Lt[tl.lev[0]];
   {
-    const T* gf = static_cast<const T*>(G0.f);
-    const T* gu = static_cast<const T*>(G0.u);
+    const sp<kG, const T> gf = gfield<const T>((const void*)G0.f);
+    const sp<kG, const T> gu = gfield<const T>((const void*)G0.u);
+    const sp<kS, T> lf = Fv(0), lu = U(0);
     for (int e = tid; e < G0.ncell; e += nthr) {
-      buf[tl.lds_f[0] + e] = gf[e];
-      if (tl.copy_u_in) buf[tl.lds_u[0] + e] = gu[e];
+      lf[e] = gf[e];
+      if (tl.copy_u_in) lu[e] = gu[e];
     }
   }
   __syncthreads();
-  auto half = [&](const DevLevel& L, int parity, int zg) {
-    if (tl.zline) {
-      const int n = L.hx * L.ny;
-      for (int e = tid; e < n; e += nthr) zline_column<T, TailNz<T>::value>(L, parity, zg, e);
+  // one routine per step type, so that each element routine is instantiated
+  // once (the kernel stays small enough for the instruction cache)
+  for (int st = 0; st < tl.nsteps; ++st) {
+    const int code = tl.steps[st];
+    const int type = code & 7, d = (code >> 3) & 15, par = (code >> 7) & 1, zg = (code >> 8) & 3;
+    const DevLevel& L = Ls[d];
+    if (type == kStepHalf) {
+      if (tl.zline) {
+        const int n = L.hx * L.ny;
+        for (int e = tid; e < n; e += nthr) zline_column<T, NZ>(L, U(d), Fv(d), CO(d), par, zg, e);
+      } else {
+        const int n = L.hx * L.ny * L.nz;
+        for (int e = tid; e < n; e += nthr) rb_point_elem<T>(L, U(d), Fv(d), CO(d), e, par, zg);
+      }
+    } else if (type == kStepRestrict) {
+      const DevXfer x = X[tl.xf[d]];
+      const DevLevel& C = Ls[d + 1];
+      for (int e = tid; e < C.ncell; e += nthr) {
+        if (ALL222) restrict_elem<T, 2, 2, 2>(L, U(d), Fv(d), CO(d), VOL(d), C, Fv(d + 1), e);
+        else restrict_any<T>(L, U(d), Fv(d), CO(d), VOL(d), C, Fv(d + 1), x, e);
+      }
+    } else if (type == kStepCoarse) {
+      for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, U(d), Fv(d), r);
+    } else if (type == kStepProlong) {
+      const DevXfer x = X[tl.xf[d]];
+      const DevLevel& C = Ls[d + 1];
+      const int nd = prolong_duals(L, C, x);
+      for (int e = tid; e < nd; e += nthr) {
+        if (ALL222) prolong_dual<T, 2, 2, 2>(L, U(d), C, U(d + 1), e);
+        else prolong_any<T>(L, U(d), C, U(d + 1), x, e);
+      }
     } else {
-      const int n = L.hx * L.ny * L.nz;
-      for (int e = tid; e < n; e += nthr) rb_point_elem<T>(L, e, parity, zg);
+      const sp<kS, T> u = U(d);
+      for (int e = tid; e < L.ncell; e += nthr) u[e] = T(0);
     }
-    __syncthreads();
-  };
-  for (int d = 0; d + 1 < nl; ++d) {
-    const DevLevel& L = Ls[d];
-    const bool zf = d > 0 || tl.zero_first;
-    if (zf && tl.pre[d] == 0) {
-      for (int e = tid; e < L.ncell; e += nthr) static_cast<T*>(L.u)[e] = T(0);
-      __syncthreads();
-    }
-    for (int s = 0; s < tl.pre[d]; ++s) {
-      half(L, 0, (zf && s == 0) ? 3 : 0);
-      half(L, 1, (zf && s == 0) ? 2 : 0);
-    }
-    const DevXfer x = X[tl.xf[d]];
-    const DevLevel& C = Ls[d + 1];
-    for (int e = tid; e < C.ncell; e += nthr) restrict_elem<T>(L, C, x, e);
     __syncthreads();
   }
-  {
-    const DevLevel& L = Ls[nl - 1];
-    for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, r);
-    __syncthreads();
-  }
-  for (int d = nl - 2; d >= 0; --d) {
-    const DevLevel& L = Ls[d];
-    const DevXfer x = X[tl.xf[d]];
-    for (int e = tid; e < L.ncell; e += nthr) prolong_elem<T>(L, Ls[d + 1], x, e);
-    __syncthreads();
-    for (int s = 0; s < tl.post[d]; ++s) {
-      half(L, 0, 0);
-      half(L, 1, 0);
-    }
-  }
-  T* gu = static_cast<T*>(G0.u);
-  for (int e = tid; e < G0.ncell; e += nthr) gu[e] = buf[tl.lds_u[0] + e];
+  const sp<kG, T> gu = gfield<T>(G0.u);
+  const sp<kS, T> lu = U(0);
+  for (int e = tid; e < G0.ncell; e += nthr) gu[e] = lu[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -701,93 +904,121 @@ __device__ __forceinline__ void fine_loop(const DevLevel& L, int lb, double* __r
   block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
 }
 
-// {max|b - A u|, max|b|}
-__global__ __launch_bounds__(kBlock) void k_residual_max(const DevLevel* __restrict__ Lt,
-                                                         LaunchList ll,
-                                                         double* __restrict__ partial) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevLevel& L = Lt[ll.items[t]];
-  const double* __restrict__ co = L.co64;
-  const double* __restrict__ u = static_cast<const double*>(L.u);
-  const double* __restrict__ f = static_cast<const double*>(L.f);
-  fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
-    int i, j, k;
-    decode(L, e, i, j, k);
-    const Nb n = neighbours(L, i, j, k);
-    const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
-    v[0] = nmax(v[0], fabs(f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c])));
-    v[1] = nmax(v[1], fabs(f[n.c]));
-  });
-}
-
 // {max|u - u2|, max|u|}
 __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restrict__ Lt,
                                                        LaunchList ll, double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const double* __restrict__ u = static_cast<const double*>(L.u);
-  const double* __restrict__ u2 = static_cast<const double*>(L.u2);
+  const auto u = gfield<const double>((const void*)L.u);
+  const auto u2 = gfield<const double>((const void*)L.u2);
   fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
     v[0] = nmax(v[0], fabs(u[e] - u2[e]));
     v[1] = nmax(v[1], fabs(u[e]));
   });
 }
 
-// r = b - A u (FP64) -> FP32 right-hand side of the correction level; {max|r|, max|b|}
-__global__ __launch_bounds__(kBlock) void k_defect32(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                     double* __restrict__ partial) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevLevel& L = Lt[ll.items[t]];
-  const double* __restrict__ co = L.co64;
-  const double* __restrict__ u = static_cast<const double*>(L.u);
-  const double* __restrict__ f = static_cast<const double*>(L.f);
-  float* __restrict__ r32 = static_cast<float*>(Lt[L.link].f);
-  fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
-    int i, j, k;
-    decode(L, e, i, j, k);
-    const Nb n = neighbours(L, i, j, k);
-    const double s = offsum<double>(co, L, i, j, k, u[n.xm], u[n.xp], u[n.ym], u[n.yp], u[n.zm], u[n.zp]);
-    const double r = f[n.c] - (s + diag_at<double>(co, L, i, j, k) * u[n.c]);
-    r32[e] = (float)r;
-    v[0] = nmax(v[0], fabs(r));
-    v[1] = nmax(v[1], fabs(f[n.c]));
-  });
+// FP64 finest-level stencil kernels, marching in z.  A thread owns one
+// position col = (j*2 + colour)*hx + m of a plane and walks kMarchK planes,
+// keeping the centre values at k-1, k, k+1 in registers: every value of the
+// iterate is read from memory once per slab (plus one halo plane at each
+// slab end), the four in-plane neighbours come from the cache, and the
+// index arithmetic is done once per thread instead of once per cell.
+//   kResMax:        {max|b - A u|, max|b|}
+//   kDefect32:      r = b - A u -> FP32 right-hand side of the correction
+//                   level; {max|r|, max|b|}
+//   kCorrDefect32:  p_out = p_in + e, r = b - A p_out -> FP32; p_in is never
+//                   written, so neighbouring threads read consistent values;
+//                   {max|r|, max|e|, max|p_out|}
+// The stencil sums are ordered as in offsum/diag_at.
+enum { kResMax = 0, kDefect32 = 1, kCorrDefect32 = 2 };
+
+// The marching loop of one thread.  Every field pointer is a __restrict__
+// parameter, so the compiler may issue the next planes' loads ahead of this
+// plane's stores (the outputs never alias the inputs).
+template <int MODE, int NV>
+__device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
+                                           sp<kG, const float> __restrict__ ev,
+                                           sp<kG, const double> __restrict__ f,
+                                           sp<kG, double> __restrict__ pout,
+                                           sp<kG, float> __restrict__ r32,
+                                           sp<kG, const double> __restrict__ cz, int nz, int plane,
+                                           int col, int oxm, int oxp, int oym, int oyp, double wxm,
+                                           double wxp, double wym, double wyp, double dxy_y,
+                                           double dxy_x, int k0, int k1, double (&v)[NV]) {
+  constexpr bool MIXED = MODE == kCorrDefect32;
+  auto val = [&](int idx) -> double {
+    if constexpr (MIXED) return u[idx] + (double)ev[idx];
+    else return u[idx];
+  };
+  const int km0 = (k0 == 0) ? nz - 1 : k0 - 1;
+  double vm = val(km0 * plane + col);
+  double vc = val(k0 * plane + col);
+#pragma unroll 4
+  for (int k = k0; k < k1; ++k) {
+    const int kp = (k == nz - 1) ? 0 : k + 1;
+    const int e = k * plane + col;
+    const double vp = val(kp * plane + col);
+    const int pb = k * plane;
+    const double xm = val(pb + oxm), xp = val(pb + oxp), ym = val(pb + oym), yp = val(pb + oyp);
+    double s = wxm * xm + wxp * xp;
+    s += wym * ym;
+    s += wyp * yp;
+    s += cz[k] * vm;
+    s += cz[nz + k] * vp;
+    const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
+    const double fb = f[e];
+    const double res = fb - (s + diag * vc);
+    if constexpr (MODE == kCorrDefect32) {
+      pout[e] = vc;
+      r32[e] = (float)res;
+      v[0] = nmax(v[0], fabs(res));
+      v[1] = nmax(v[1], fabs((double)ev[e]));
+      v[2] = nmax(v[2], fabs(vc));
+    } else {
+      if constexpr (MODE == kDefect32) r32[e] = (float)res;
+      v[0] = nmax(v[0], fabs(res));
+      v[1] = nmax(v[1], fabs(fb));
+    }
+    vm = vc;
+    vc = vp;
+  }
 }
 
-// p_out = p_in + e, then r = b - A p_out from p_in and e at the seven points
-// (p_in is never written, so neighbouring threads read consistent values);
-// {max|r|, max|e|, max|p_out|}
-__global__ __launch_bounds__(kBlock) void k_correct_defect32(const DevLevel* __restrict__ Lt,
-                                                             LaunchList ll,
-                                                             double* __restrict__ partial) {
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                 double* __restrict__ partial) {
+  constexpr int NV = MODE == kCorrDefect32 ? 3 : 2;
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const DevLevel& S = Lt[L.link];
-  const double* __restrict__ co = L.co64;
-  const double* __restrict__ p = static_cast<const double*>(L.u);
-  const float* __restrict__ ev = static_cast<const float*>(S.u);
-  const double* __restrict__ bb = static_cast<const double*>(L.f);
-  double* __restrict__ pout = static_cast<double*>(L.u2);
-  float* __restrict__ r32 = static_cast<float*>(S.f);
-  fine_loop<3>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
-    int i, j, k;
-    decode(L, e, i, j, k);
-    const Nb n = neighbours(L, i, j, k);
-#define PN(q) (p[n.q] + (double)ev[n.q])
-    const double pc = PN(c);
-    const double s = offsum<double>(co, L, i, j, k, PN(xm), PN(xp), PN(ym), PN(yp), PN(zm), PN(zp));
-#undef PN
-    const double r = bb[n.c] - (s + diag_at<double>(co, L, i, j, k) * pc);
-    pout[n.c] = pc;
-    r32[n.c] = (float)r;
-    v[0] = nmax(v[0], fabs(r));
-    v[1] = nmax(v[1], fabs((double)ev[n.c]));
-    v[2] = nmax(v[2], fabs(pc));
-  });
+  const int hx = L.hx, nx = L.nx, ny = L.ny, nz = L.nz;
+  const int plane = 2 * hx * ny;
+  const int ncb = (plane + kBlock - 1) / kBlock;
+  const int lb = b - ll.prefix[t];
+  const int bk = lb / ncb, bc = lb - bk * ncb;
+  const int col = bc * kBlock + threadIdx.x;
+  const int k0 = bk * kMarchK, k1 = min(k0 + kMarchK, nz);
+  double v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = 0.0;
+  if (col < plane) {
+    const int m = col % hx, row = col / hx, c = row & 1, j = row >> 1;
+    const int i = 2 * m + ((c + j) & 1), r = i & 1, oth = c ^ 1;
+    int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
+    int mr = m + r;     if (mr >= hx) mr = 0;
+    const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+    const auto co = gp(L.co64);
+    const auto cy = co + 3 * nx;
+    const auto cz = cy + 3 * ny;
+    fine_march<MODE, NV>(gfield<const double>((const void*)L.u), gfield<const float>((const void*)S.u),
+                         gfield<const double>((const void*)L.f), gfield<double>(L.u2),
+                         gfield<float>(S.f), cz, nz, plane, col, (j * 2 + oth) * hx + ml,
+                         (j * 2 + oth) * hx + mr, (jm * 2 + oth) * hx + m, (jp * 2 + oth) * hx + m,
+                         co[i], co[nx + i], cy[j], cy[ny + j], cy[2 * ny + j], co[2 * nx + i], k0, k1, v);
+  }
+  block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
 }
 
 // One block per item: thread t folds partials b0+t, b0+t+256, ... in order,
@@ -833,12 +1064,12 @@ __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restr
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
-  double v = ptr.p[t][fidx(i, j, k, L.nx, L.ny)];
+  double v = gp(ptr.p[t])[fidx(i, j, k, L.nx, L.ny)];
   if (add_fold) {
-    const double* bf = L.bcfold;
+    const auto bf = gp(L.bcfold);
     v += (bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k];
   }
-  static_cast<double*>(which == 0 ? L.u : L.f)[e] = v;
+  gfield<double>(which == 0 ? L.u : L.f)[e] = v;
 }
 
 // Weighted sums: each thread folds kWsumPer cells (in a fixed order), one
@@ -849,16 +1080,17 @@ __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
+  const auto vol = gp(L.vol);
+  const auto rho = gp(L.rho);
+  const auto x = gfield<const double>((const void*)(which == 0 ? L.u : L.f));
   double v[1] = {0.0};
   for (int s = 0; s < kWsumPer; ++s) {
     const int e = e0 + s * kBlock;
     if (e < L.ncell) {
       int i, j, k;
       decode(L, e, i, j, k);
-      const double w = L.vol[L.nx + L.ny + k] * L.vol[L.nx + j] * L.vol[i];
-      const double x = (which == 2) ? L.rho[k]
-                                    : static_cast<const double*>(which == 0 ? L.u : L.f)[e];
-      v[0] += w * x;
+      const double w = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
+      v[0] += w * ((which == 2) ? rho[k] : x[e]);
     }
   }
   block_reduce_store<1, true>(v, partial + b);
@@ -877,12 +1109,12 @@ __global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ L
   const DevLevel& L = Lt[ll.items[t]];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
-  double* x = static_cast<double*>(which == 0 ? L.u : L.f);
+  const auto x = gfield<double>(which == 0 ? L.u : L.f);
   double s = c[t] / (with_rho ? L.vrsum : L.vsum);
   if (with_rho) {
     int i, j, k;
     decode(L, e, i, j, k);
-    s *= L.rho[k];
+    s *= gp(L.rho)[k];
   }
   x[e] -= s;
 }
@@ -896,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restric
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
-  ptr.p[t][fidx(i, j, k, L.nx, L.ny)] = static_cast<const double*>(L.u)[e];
+  gp(ptr.p[t])[fidx(i, j, k, L.nx, L.ny)] = gfield<const double>((const void*)L.u)[e];
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ Lt, LaunchList ll,
@@ -907,7 +1139,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ 
   if (active && !active[L.inst]) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
-  static_cast<double*>(L.u2)[e] = static_cast<const double*>(L.u)[e];
+  gfield<double>(L.u2)[e] = gfield<const double>((const void*)L.u)[e];
 }
 
 template <class T>
@@ -916,7 +1148,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(const DevLevel* __restrict
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e < L.ncell) static_cast<T*>(L.u)[e] = T(0);
+  if (e < L.ncell) gfield<T>(L.u)[e] = T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1093,11 +1325,13 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                   int segs, hipStream_t s) {
   const dim3 blk(64, segs);
-  const size_t lds = (size_t)6 * segs * 64 * (fp64 ? 8 : 4);
+  const size_t lds = (size_t)(6 * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
   if (fp64)
-    hipLaunchKernelGGL((k_zline<double, kZSeg>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess);
+    hipLaunchKernelGGL((k_zline<double, kZSeg, kZChunk64>), dim3(ll.nblocks), blk, lds, s, L, ll, parity,
+                       zero_guess);
   else
-    hipLaunchKernelGGL((k_zline<float, kZSeg>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess);
+    hipLaunchKernelGGL((k_zline<float, kZSeg, kZChunk32>), dim3(ll.nblocks), blk, lds, s, L, ll, parity,
+                       zero_guess);
 }
 void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                       int zero_guess, hipStream_t s) {
@@ -1118,11 +1352,27 @@ void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream
   if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll);
   else LAUNCH(k_coarse<float>, ll.nblocks, kBlock, L, ll);
 }
-void launch_tail(bool fp64, const DevLevel* L, const DevXfer* X, const DevTail* T,
-                 const LaunchList& ll, size_t lds_bytes, hipStream_t s) {
-  const dim3 blk(64, kTailWaves);
-  if (fp64) hipLaunchKernelGGL(k_tail<double>, dim3(ll.nitems), blk, lds_bytes, s, L, X, T, ll);
-  else hipLaunchKernelGGL(k_tail<float>, dim3(ll.nitems), blk, lds_bytes, s, L, X, T, ll);
+int tail_nz_class(bool fp64, int nz_max) {
+  if (nz_max <= 8) return 8;
+  if (nz_max <= 16 || fp64) return 16;
+  return 32;
+}
+
+#define TAIL_KERNEL(T, NZ, F) (k_tail<T, NZ, F>)
+void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
+                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s) {
+  const dim3 blk(64, kTailWaves), grd(ll.nitems);
+#define GO(TT, NZ)                                                                         \
+  do {                                                                                     \
+    if (all222) hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, true), grd, blk, lds_bytes, s, L, X, T, ll);  \
+    else hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, false), grd, blk, lds_bytes, s, L, X, T, ll);        \
+  } while (0)
+  if (fp64) {
+    if (nz_class <= 8) GO(double, 8); else GO(double, 16);
+  } else {
+    if (nz_class <= 8) GO(float, 8); else if (nz_class <= 16) GO(float, 16); else GO(float, 32);
+  }
+#undef GO
 }
 
 size_t tail_lds_limit() {
@@ -1136,10 +1386,21 @@ size_t tail_lds_limit() {
     lim = (size_t)v;
   // room for the static descriptor array of the kernel
   lim -= sizeof(DevLevel) * kTailMaxLevels + 256;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<double>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess ||
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tail<float>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
+  const void* kernels[] = {
+      reinterpret_cast<const void*>(&k_tail<double, 8, true>),
+      reinterpret_cast<const void*>(&k_tail<double, 8, false>),
+      reinterpret_cast<const void*>(&k_tail<double, 16, true>),
+      reinterpret_cast<const void*>(&k_tail<double, 16, false>),
+      reinterpret_cast<const void*>(&k_tail<float, 8, true>),
+      reinterpret_cast<const void*>(&k_tail<float, 8, false>),
+      reinterpret_cast<const void*>(&k_tail<float, 16, true>),
+      reinterpret_cast<const void*>(&k_tail<float, 16, false>),
+      reinterpret_cast<const void*>(&k_tail<float, 32, true>),
+      reinterpret_cast<const void*>(&k_tail<float, 32, false>)};
+  bool ok = true;
+  for (const void* k : kernels)
+    ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) == hipSuccess;
+  if (!ok) {
     (void)hipGetLastError();
     lim = 64 * 1024 - sizeof(DevLevel) * kTailMaxLevels - 256;
   }
@@ -1147,17 +1408,17 @@ size_t tail_lds_limit() {
   return limit;
 }
 void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH(k_residual_max, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH(k_fine<kResMax>, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
   LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH(k_defect32, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH(k_fine<kDefect32>, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
                              hipStream_t s) {
-  LAUNCH(k_correct_defect32, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH(k_fine<kCorrDefect32>, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);

@@ -229,11 +229,17 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     // grow the tail upward from the coarsest level while it stays small,
     // short-columned and inside the LDS budget
     const size_t limit = tail_lds_limit();
+    auto al = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    // LDS of one tail level: u, f, coefficient table, cell widths
+    auto level_bytes = [&](const HostLevel& H, bool coarsest) {
+      const size_t n1 = (size_t)H.nx + H.ny + H.nz;
+      return 2 * al((size_t)H.ncell() * w) + al(3 * n1 * w) + (coarsest ? 0 : al(n1 * 8));
+    };
     int l0 = L - 1;
-    size_t bytes = 2 * (size_t)I->H.lv[L - 1].ncell() * w;
+    size_t bytes = level_bytes(I->H.lv[L - 1], true);
     for (int l = L - 2; l >= 0; --l) {
       const HostLevel& H = I->H.lv[l];
-      const size_t nb2 = bytes + 2 * (size_t)H.ncell() * w;
+      const size_t nb2 = bytes + level_bytes(H, false);
       if (H.ncell() > kTailCells || nb2 > limit) break;
       if (cfg.smoother == AMG_SMOOTHER_LJ && H.nz > (I->fp64 ? kTailNz64 : kTailNz)) break;
       l0 = l;
@@ -248,22 +254,71 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     DevTail tl;
     std::memset(&tl, 0, sizeof tl);
     tl.nlev = L - l0;
-    tl.zero_first = (l0 > 0 || !I->fp64) ? 1 : 0;
-    tl.copy_u_in = tl.zero_first ? 0 : 1;
+    const bool zero_first = l0 > 0 || !I->fp64;   // the first tail level starts from u = 0
+    tl.copy_u_in = zero_first ? 0 : 1;
     tl.zline = cfg.smoother == AMG_SMOOTHER_LJ ? 1 : 0;
-    int off = 0;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+      const size_t o = off;
+      off += al(b);
+      return (int)o;
+    };
     for (int d = 0; d < tl.nlev; ++d) {
+      const HostLevel& H = I->H.lv[l0 + d];
+      const size_t n1 = (size_t)H.nx + H.ny + H.nz;
       tl.lev[d] = I->lev[l0 + d];
       if (l0 + d < L - 1) tl.xf[d] = I->xf[l0 + d];
-      tl.pre[d] = I->pre[l0 + d];
-      tl.post[d] = I->post[l0 + d];
-      const int nc = (int)I->H.lv[l0 + d].ncell();
-      tl.lds_u[d] = off;
-      tl.lds_f[d] = off + nc;
-      off += 2 * nc;
+      tl.lds_u[d] = take((size_t)H.ncell() * w);
+      tl.lds_f[d] = take((size_t)H.ncell() * w);
+      tl.lds_co[d] = take(3 * n1 * w);
+      tl.lds_vol[d] = (d + 1 < tl.nlev) ? take(n1 * 8) : 0;
     }
+    // the sub-V-cycle as a step list, in the order of mgMK/mg_cycle.F90
+    // (pre-smoothing and restriction down, the coarsest solve, prolongation
+    // and post-smoothing up); from a zero guess the first red pass sees
+    // zeros everywhere and the first black pass zero old black values
+    {
+      std::vector<int> st;
+      auto step = [&](int type, int d, int par, int zg) {
+        st.push_back(type | (d << 3) | (par << 7) | (zg << 8));
+      };
+      for (int d = 0; d + 1 < tl.nlev; ++d) {
+        const bool zf = d > 0 || zero_first;
+        const int pre = I->pre[l0 + d];
+        if (zf && pre == 0) step(kStepZero, d, 0, 0);
+        for (int q = 0; q < pre; ++q) {
+          step(kStepHalf, d, 0, (zf && q == 0) ? 3 : 0);
+          step(kStepHalf, d, 1, (zf && q == 0) ? 2 : 0);
+        }
+        step(kStepRestrict, d, 0, 0);
+      }
+      step(kStepCoarse, tl.nlev - 1, 0, 0);
+      for (int d = tl.nlev - 2; d >= 0; --d) {
+        step(kStepProlong, d, 0, 0);
+        for (int q = 0; q < I->post[l0 + d]; ++q) {
+          step(kStepHalf, d, 0, 0);
+          step(kStepHalf, d, 1, 0);
+        }
+      }
+      if ((int)st.size() > kTailMaxSteps) {
+        set_error("amg_create: too many smoothing sweeps on the coarse levels");
+        delete I;
+        return AMG_EINVAL;
+      }
+      tl.nsteps = (int)st.size();
+      std::copy(st.begin(), st.end(), tl.steps);
+    }
+    int nzmax = 1;
+    bool all222 = true;
+    for (int d = 0; d < tl.nlev; ++d) {
+      const HostLevel& H = I->H.lv[l0 + d];
+      nzmax = std::max(nzmax, H.nz);
+      if (d + 1 < tl.nlev && (H.fx != 2 || H.fy != 2 || H.fz != 2)) all222 = false;
+    }
+    I->tail_nzc = tail_nz_class(I->fp64, nzmax);
+    I->tail_222 = all222;
     I->tail = (int)hT.size();
-    I->tail_lds = (size_t)off * w;
+    I->tail_lds = off;
     hT.push_back(tl);
   }
   // projection metrics
@@ -388,21 +443,28 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
       launch_restrict(fp64, L, dX, ll, stream);
     });
   }
-  {
-    std::vector<Item> items;
-    size_t lds = 0;
-    for (Instance* I : S) {
-      items.push_back({I->tail, 1});
-      lds = std::max(lds, I->tail_lds);
+  // the coarse tails, one launch per kernel variant
+  for (int nzc : {8, 16, 32}) {
+    for (int a222 = 0; a222 < 2; ++a222) {
+      std::vector<Item> items;
+      size_t lds = 0;
+      for (Instance* I : S) {
+        if (I->tail_nzc != nzc || (int)I->tail_222 != a222) continue;
+        items.push_back({I->tail, 1});
+        lds = std::max(lds, I->tail_lds);
+      }
+      for_lists(items, [&](const LaunchList& ll, int, int) {
+        launch_tail(fp64, nzc, a222 != 0, L, dX, dT, ll, lds, stream);
+      });
     }
-    for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_tail(fp64, L, dX, dT, ll, lds, stream);
-    });
   }
   for (int dpt = maxL0 - 1; dpt >= 0; --dpt) {
     std::vector<Item> items;
     for (Instance* I : S)
-      if (dpt < I->l0) items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
+      if (dpt < I->l0)
+        items.push_back({I->xf[dpt], cdiv(prolong_duals(hL[I->lev[dpt]], hL[I->lev[dpt + 1]],
+                                                        hX[I->xf[dpt]]),
+                                          kBlock)});
     for_lists(items, [&](const LaunchList& ll, int, int) {
       launch_prolong(fp64, L, dX, ll, stream);
     });
@@ -420,6 +482,17 @@ std::vector<Item> fine_items(const Context& C, const std::vector<Instance*>& S, 
   }
   return items;
 }
+
+// Items of the z-marching FP64 stencil kernels (residual, defect, correct+defect).
+std::vector<Item> march_items(const Context& C, const std::vector<Instance*>& S, bool current) {
+  std::vector<Item> items;
+  for (Instance* I : S) {
+    const int li = current ? I->fine[I->cur] : I->fine[0];
+    const DevLevel& d = C.hL[li];
+    items.push_back({li, (int)march_blocks(d.hx, d.ny, d.nz)});
+  }
+  return items;
+}
 }  // namespace
 
 // Launches one cycle on every instance of S plus the norm kernels and the
@@ -433,22 +506,26 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S) {
     for_lists(items, [&](const LaunchList& ll, int, int) { launch_copy_u(dL, ll, nullptr, stream); });
   }
   vcycle(S, fp64);
-  auto items = fine_items(*this, S, true, (long long)kBlock * kFinePer);
+  auto items = march_items(*this, S, true);
   const int nv = 4;
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
     if (fp64) {
       launch_residual_max(dL, ll, part, stream);
       launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
-      if (relmax) {
-        launch_change_max(dL, ll, part, stream);
-        launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
-      }
     } else {
       launch_correct_defect32(dL, ll, part, stream);
       launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream);
     }
   });
+  if (relmax) {
+    auto citems = fine_items(*this, S, true, (long long)kBlock * kFinePer);
+    for_lists(citems, [&](const LaunchList& ll, int ioff, int boff) {
+      double* part = d_partial + (size_t)boff * nv;
+      launch_change_max(dL, ll, part, stream);
+      launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
+    });
+  }
   (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
   if (relmax)
     (void)hipMemcpyAsync(h_out + kOutRegion * 2, d_out + kOutRegion * 2, sizeof(double) * S.size() * 2,
@@ -548,7 +625,7 @@ int Context::prepare(const std::vector<Instance*>& S) {
   remove_mean(S, true);
   // initial residual (mixed: also the FP32 defect of the first cycle)
   const bool fp64 = S[0]->fp64;
-  const auto fitems = fine_items(*this, S, false, (long long)kBlock * kFinePer);
+  const auto fitems = march_items(*this, S, false);
   for_lists(fitems, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
     if (fp64) launch_residual_max(dL, ll, part, stream);
@@ -702,11 +779,13 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
       set_error("amg_run_kernel: no coarser level");
       return AMG_EINVAL;
     }
-    const int nc = which == 1 ? hL[I->lev[level + 1]].ncell : d.ncell;
+    const int nc = which == 1 ? hL[I->lev[level + 1]].ncell
+                              : prolong_duals(d, hL[I->lev[level + 1]], hX[I->xf[level]]);
     items.push_back({I->xf[level], cdiv(nc, kBlock)});
     b = (2.0 + 0.125) * w * d.ncell;
   } else if (which == 3) {
-    items.push_back({I->fine[I->cur], cdiv(hL[I->fine[I->cur]].ncell, (long long)kBlock * kFinePer)});
+    const DevLevel& fd = hL[I->fine[I->cur]];
+    items.push_back({I->fine[I->cur], (int)march_blocks(fd.hx, fd.ny, fd.nz)});
     b = fp64 ? 16.0 * d.ncell : 32.0 * d.ncell;
   } else if (which == 7 || which == 8) {
     if (!zline || d.nz > (fp64 ? kTailNz64 : kTailNz)) {

@@ -53,6 +53,8 @@ struct Instance {
   int l0 = 0;                    // first level of the coarse tail
   int tail = -1;                 // DevTail index
   size_t tail_lds = 0;           // dynamic LDS bytes of the tail
+  int tail_nzc = 16;             // tail kernel variant: column length class
+  bool tail_222 = true;          //   and uniform 2:1 transfers
   int fine[2] = {-1, -1};        // FP64 finest entries (u = p_cur for cur = 0/1)
   int cur = 0;                   // which fine entry holds the current pressure
   std::vector<int> pre, post;
