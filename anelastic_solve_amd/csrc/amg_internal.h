@@ -94,6 +94,7 @@ constexpr int kTailMaxLevels = 16;
 // code = type | level << 3 | parity << 7 | zero_guess << 8.
 enum { kStepZero = 0, kStepHalf = 1, kStepRestrict = 2, kStepCoarse = 3, kStepProlong = 4 };
 constexpr int kTailMaxSteps = 1024;
+constexpr size_t kTailMinvLds = 16384;   // largest coarsest inverse staged in LDS (bytes)
 struct DevTail {
   int nlev;                       // tail levels, the last is the coarsest
   int copy_u_in;                  // the first tail level's u is read (not zero)
@@ -107,6 +108,7 @@ struct DevTail {
   int lds_f[kTailMaxLevels];
   int lds_co[kTailMaxLevels];
   int lds_vol[kTailMaxLevels];
+  int lds_minv;                   // the coarsest inverse staged in LDS (level precision), or -1
   int lev[kTailMaxLevels];        // DevLevel index per tail level
   int xf[kTailMaxLevels];         // DevXfer index of tail level d -> d+1
 };

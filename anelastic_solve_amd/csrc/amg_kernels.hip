@@ -251,7 +251,7 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   constexpr bool LDS_COEF = sizeof(T) == 8;
   constexpr int SR = LDS_COEF ? 1 : S;
   T A[SR], E[SR], Dz[SR];
-  T* const cA = lds + 6 * P * 64 + p * 3 * S;
+  T* const cA = lds + (sizeof(T) == 8 ? 6 : 12) * P * 64 + p * 3 * S;
   T* const cE = cA + S;
   T* const cD = cE + S;
   T za = T(0), ze = T(0), zd = T(0);
@@ -322,8 +322,11 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
 #pragma unroll
   for (int q = 0; q < S; ++q) c_last = (q == len - 1) ? E(q) : c_last;
   // phase 1: interface data of segment p
+  // The eliminations are written for a segment of n rows; a full segment
+  // (n == S, every segment but the last of a ragged column) instantiates
+  // them with a constant n, so the unrolled chains carry no branches.
   T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
-  if (len > 0) {
+  auto interface_data = [&](const int len) {
     // LU sweep: yl, vl, wl, w0
     T cp = T(0), yf = T(0), vf = T(0), prod = T(1);
 #pragma unroll
@@ -361,53 +364,63 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     }
     y0 = h;
     v0 = a_first * ri;
-  }
+  };
+  if (sizeof(T) == 4 && len == S) interface_data(S);
+  else if (len > 0) interface_data(len);
   // phase 2: reduced system X_p = (t_p, b_p) (first/last values of segment p)
   //   t_p + v0_p b_{p-1} + w0_p t_{p+1} = y0_p
   //   b_p + vl_p b_{p-1} + wl_p t_{p+1} = yl_p
-  T* Y0 = lds;
-  T* YL = Y0 + P * 64;
-  T* V0 = YL + P * 64;
-  T* VL = V0 + P * 64;
-  T* W0 = VL + P * 64;
-  T* WL = W0 + P * 64;
+  // solved by parallel cyclic reduction over the segments: at distance d
+  // every segment substitutes b_{p-d} (row b of p-d) and t_{p+d} (row t of
+  // p+d); the couplings keep their shape (to b_{p-2d} and t_{p+2d} only), so
+  // after ceil(log2 P) steps every segment holds its own values.  Segments
+  // past either end of the column contribute zeros (the end couplings are
+  // zero: walls, and periodic wraps are lagged).  Every wave takes part.
+  // FP32 double-buffers the state in LDS (one barrier per step); FP64 uses
+  // one buffer and two barriers, keeping the block within 64 KiB of LDS.
   const int sl = p * 64 + lane;
   T bprev = T(0), tnext = T(0);
+  constexpr int kZBuf = sizeof(T) == 8 ? 1 : 2;
   if (Pi > 1) {
-    Y0[sl] = y0; YL[sl] = yl; V0[sl] = v0; VL[sl] = vl; W0[sl] = w0; WL[sl] = wl;
-    __syncthreads();
-    if (p == 0) {
-      // downward: b_{q-1} = beta - gamma t_q ;  t_q = tau - sigma t_{q+1}
-      T beta = T(0), gamma = T(0);
-      for (int q = 0; q < Pi; ++q) {
-        const int s2 = q * 64 + lane;
-        const T v0 = V0[s2], y0q = Y0[s2], w0 = W0[s2], vl = VL[s2], yl = YL[s2], wl = WL[s2];
-        const T dn = rcp(T(1) - v0 * gamma);
-        const T tau = (y0q - v0 * beta) * dn;
-        const T sigma = w0 * dn;
-        const T nb = yl - vl * beta + vl * gamma * tau;
-        const T ng = wl + vl * gamma * sigma;
-        V0[s2] = tau; W0[s2] = sigma; VL[s2] = nb; WL[s2] = ng;
-        beta = nb; gamma = ng;
-      }
-      // upward: t_P = 0
-      T tn = T(0);
-      for (int q = Pi - 1; q >= 0; --q) {
-        const int s2 = q * 64 + lane;
-        const T tq = V0[s2] - W0[s2] * tn;
-        const T bq = VL[s2] - WL[s2] * tn;
-        Y0[s2] = tq;
-        YL[s2] = bq;
-        tn = tq;
-      }
+    const int PS = P * 64;
+    T at = v0, ab = vl, ct = w0, cb = wl, rt = y0, rb = yl;
+    int buf = 0;
+    for (int d = 1; d < Pi; d <<= 1, buf ^= (kZBuf - 1)) {
+      T* W = lds + buf * 6 * PS;
+      W[sl] = at; W[PS + sl] = ab; W[2 * PS + sl] = ct;
+      W[3 * PS + sl] = cb; W[4 * PS + sl] = rt; W[5 * PS + sl] = rb;
+      __syncthreads();
+      const bool hm = p - d >= 0, hp = p + d < Pi;
+      const int sm = hm ? (p - d) * 64 + lane : sl, spp = hp ? (p + d) * 64 + lane : sl;
+      // row b of segment p-d, row t of segment p+d
+      const T abm = W[PS + sm], cbm = W[3 * PS + sm], rbm = W[5 * PS + sm];
+      const T atp = W[spp], ctp = W[2 * PS + spp], rtp = W[4 * PS + spp];
+      const T ab_m = hm ? abm : T(0), cb_m = hm ? cbm : T(0), rb_m = hm ? rbm : T(0);
+      const T at_p = hp ? atp : T(0), ct_p = hp ? ctp : T(0), rt_p = hp ? rtp : T(0);
+      if (kZBuf == 1) __syncthreads();   // every read done before the next writes
+      const T m11 = T(1) - at * cb_m, m12 = -ct * at_p;
+      const T m21 = -ab * cb_m, m22 = T(1) - cb * at_p;
+      const T id = rcp(m11 * m22 - m12 * m21);
+      const T qt = rt - at * rb_m - ct * rt_p, qb = rb - ab * rb_m - cb * rt_p;
+      const T lt = -at * ab_m, lb = -ab * ab_m;     // couplings to b_{p-2d}
+      const T ut = -ct * ct_p, ub = -cb * ct_p;     // couplings to t_{p+2d}
+      at = id * (m22 * lt - m12 * lb);
+      ab = id * (m11 * lb - m21 * lt);
+      ct = id * (m22 * ut - m12 * ub);
+      cb = id * (m11 * ub - m21 * ut);
+      rt = id * (m22 * qt - m12 * qb);
+      rb = id * (m11 * qb - m21 * qt);
     }
+    T* W = lds + buf * 6 * PS;
+    W[sl] = rt;
+    W[PS + sl] = rb;
     __syncthreads();
-    bprev = (p > 0) ? YL[(p - 1) * 64 + lane] : T(0);
-    tnext = (p < Pi - 1) ? Y0[(p + 1) * 64 + lane] : T(0);
+    bprev = (p > 0) ? W[PS + (p - 1) * 64 + lane] : T(0);
+    tnext = (p < Pi - 1) ? W[(p + 1) * 64 + lane] : T(0);
     __syncthreads();   // the group's LDS may be rewritten by its next call
   }
   // phase 3: re-solve the segment with its end couplings on the right-hand side
-  if (len > 0) {
+  auto resolve = [&](const int len) {
     T cpa[S];
     T cp = T(0), yf = T(0);
 #pragma unroll
@@ -432,7 +445,9 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
         if (live) u[(ks + q) * plane + own] = x;
       }
     }
-  }
+  };
+  if (sizeof(T) == 4 && len == S) resolve(S);
+  else if (len > 0) resolve(len);
 #undef A
 #undef E
 #undef D
@@ -457,11 +472,14 @@ template <> struct TailNz<double> { static constexpr int value = kTailNz64; };
 
 // Short columns (nz <= NZ) inside the coarse tail: one thread per column,
 // the whole column in registers, plain Thomas (the reference's line solve,
-// mgMK/mg_relax.F90:541-595); column index col = j*hx + m.
-template <class T, int NZ, class UP, class FP, class CP>
+// mgMK/mg_relax.F90:541-595); column index col = j*hx + m.  Every load (the
+// right-hand-side data and the z coefficients) is issued before the
+// elimination; with EXACT (nz == NZ) the chains carry no predicates.
+template <class T, int NZ, bool EXACT, class UP, class FP, class CP>
 __device__ __forceinline__ void zline_column(const DevLevel& L, UP u, FP f, CP co, int parity,
                                              int zero_guess, int col) {
-  const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
+  const int hx = L.hx, ny = L.ny, nx = L.nx;
+  const int nz = EXACT ? NZ : L.nz;
   const int m = col % hx, j = col / hx;
   const int c = parity, oth = parity ^ 1;
   const int i = 2 * m + ((c + j) & 1), r = i & 1;
@@ -479,32 +497,41 @@ __device__ __forceinline__ void zline_column(const DevLevel& L, UP u, FP f, CP c
   const int no = cidx(m, jp, 0, oth, hx, ny);
   const bool pz = L.pz != 0;
   const bool lag = pz && !(zero_guess & 2);
-  const T lagb = lag ? cz[0] * u[(nz - 1) * plane + own] : T(0);
-  const T lagt = lag ? cz[2 * nz - 1] * u[own] : T(0);
-  T g[NZ], cpa[NZ];
+  const bool nbr = !(zero_guess & 1);
+  const T ubot = u[(nz - 1) * plane + own], utop = u[own];
+  const T lagb = lag ? cz[0] * ubot : T(0);
+  const T lagt = lag ? cz[2 * nz - 1] * utop : T(0);
+  // z coefficients held in registers for columns of up to 16 cells
+  constexpr bool HOIST = NZ <= 16;
+  constexpr int NH = HOIST ? NZ : 1;
+  T g[NZ], za[NH], ze[NH], zd[NH];
 #pragma unroll
   for (int q = 0; q < NZ; ++q) {
-    if (q < nz) {
-      const int kb = q * plane;
-      T v = f[kb + own];
-      if (!(zero_guess & 1)) {
-        T s = wxl * u[kb + xo + ml] + wxr * u[kb + xo + mr];
-        s += wys * u[kb + so];
-        s += wyn * u[kb + no];
-        v = v - s;
-      }
-      v -= (q == 0) ? lagb : T(0);
-      v -= (q == nz - 1) ? lagt : T(0);
-      g[q] = v;
+    const int k = EXACT ? q : min(q, nz - 1);     // rows clamped: every load in bounds
+    const int kb = k * plane;
+    const T fv = f[kb + own];
+    const T xl = u[kb + xo + ml], xr = u[kb + xo + mr], ys = u[kb + so], yn = u[kb + no];
+    if constexpr (HOIST) {
+      za[q] = cz[k];
+      ze[q] = cz[nz + k];
+      zd[q] = cz[2 * nz + k];
     }
+    T sn = wxl * xl + wxr * xr;
+    sn += wys * ys;
+    sn += wyn * yn;
+    T v = nbr ? fv - sn : fv;
+    v -= (q == 0) ? lagb : T(0);
+    v -= (q == nz - 1) ? lagt : T(0);
+    g[q] = v;
   }
+  T cpa[NZ];
   T cp = T(0), yf = T(0);
 #pragma unroll
   for (int q = 0; q < NZ; ++q) {
-    if (q < nz) {
-      const T a = (q == 0) ? T(0) : cz[q];
-      const T e = (q == nz - 1) ? T(0) : cz[nz + q];
-      const T rd = rcp((cz[2 * nz + q] + dxy) - a * cp);
+    if (EXACT || q < nz) {
+      const T a = (q == 0) ? T(0) : (HOIST ? za[HOIST ? q : 0] : cz[q]);
+      const T e = (q == nz - 1) ? T(0) : (HOIST ? ze[HOIST ? q : 0] : cz[nz + q]);
+      const T rd = rcp(((HOIST ? zd[HOIST ? q : 0] : cz[2 * nz + q]) + dxy) - a * cp);
       yf = (g[q] - a * yf) * rd;
       cp = e * rd;
       g[q] = yf;
@@ -514,22 +541,31 @@ __device__ __forceinline__ void zline_column(const DevLevel& L, UP u, FP f, CP c
   T x = T(0);
 #pragma unroll
   for (int q = NZ - 1; q >= 0; --q) {
-    if (q < nz) {
+    if (EXACT || q < nz) {
       x = g[q] - cpa[q] * x;
       u[q * plane + own] = x;
     }
   }
 }
 
-// Dispatch on the column length: the smallest unrolled length that holds nz.
+// Dispatch on the column length: an exact instantiation for full columns
+// of NZ, NZ/2 and NZ/4 cells, the predicated one otherwise.
+template <class T, int NZ, class UP, class FP, class CP>
+__device__ __forceinline__ void zline_column_n(const DevLevel& L, UP u, FP f, CP co, int parity,
+                                               int zero_guess, int col) {
+  const int nz = L.nz;
+  if (nz == NZ) zline_column<T, NZ, true>(L, u, f, co, parity, zero_guess, col);
+  else if (NZ >= 8 && nz == NZ / 2) zline_column<T, (NZ >= 8 ? NZ / 2 : NZ), true>(L, u, f, co, parity, zero_guess, col);
+  else if (NZ >= 16 && nz == NZ / 4) zline_column<T, (NZ >= 16 ? NZ / 4 : NZ), true>(L, u, f, co, parity, zero_guess, col);
+  else zline_column<T, NZ, false>(L, u, f, co, parity, zero_guess, col);
+}
+
+// The standalone short-column sweep (tests of the tail's line solve).
 template <class T, class UP, class FP, class CP>
 __device__ __forceinline__ void zline_column_any(const DevLevel& L, UP u, FP f, CP co, int parity,
                                                  int zero_guess, int col) {
-  const int nz = L.nz;
-  if (nz <= 4) zline_column<T, 4>(L, u, f, co, parity, zero_guess, col);
-  else if (nz <= 8) zline_column<T, 8>(L, u, f, co, parity, zero_guess, col);
-  else if (nz <= 16 || TailNz<T>::value <= 16) zline_column<T, 16>(L, u, f, co, parity, zero_guess, col);
-  else zline_column<T, TailNz<T>::value>(L, u, f, co, parity, zero_guess, col);
+  if (L.nz <= 16 || TailNz<T>::value <= 16) zline_column_n<T, 16>(L, u, f, co, parity, zero_guess, col);
+  else zline_column_n<T, TailNz<T>::value>(L, u, f, co, parity, zero_guess, col);
 }
 
 template <class T>
@@ -768,10 +804,10 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
 // Coarsest level: u = Minv f with a precomputed FP64 (pseudo-)inverse
 // stored transposed (row r of the product reads column r: coalesced), FP64
 // accumulation in eight independent partial sums.
-template <class T, class UP, class FP>
-__device__ __forceinline__ void coarse_row(const DevLevel& L, UP u, FP f, int r) {
+template <class T, class UP, class FP, class MP>
+__device__ __forceinline__ void coarse_row(const DevLevel& L, UP u, FP f, MP Mt, int r) {
   const int n = L.ncell;
-  const sp<kG, const double> M = gp(L.minvT) + r;
+  const auto M = Mt + r;
   double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int s = 0;
   for (; s + 8 <= n; s += 8) {
@@ -789,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   const int r = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (r < L.ncell) coarse_row<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), r);
+  if (r < L.ncell) coarse_row<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), gp(L.minvT), r);
 }
 
 // ---------------------------------------------------------------------------
@@ -818,18 +854,27 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
   auto CO = [&](int d) { return (sp<kS, const T>)(base + tl.lds_co[d]); };
   auto VOL = [&](int d) { return (sp<kS, const double>)(base + tl.lds_vol[d]); };
   if (tid < nl) Ls[tid] = Lt[tl.lev[tid]];
-  // stage the tables of every tail level and the first level's fields
-  for (int d = 0; d < nl; ++d) {
+  // stage the tables of every tail level (one wave per level, in parallel),
+  // the coarsest inverse when it fits, and the first level's fields
+  const int wv = threadIdx.y, ln = threadIdx.x;
+  for (int d = wv; d < nl; d += blockDim.y) {
     const DevLevel& G = Lt[tl.lev[d]];
     const int nco = 3 * (G.nx + G.ny + G.nz), nv = G.nx + G.ny + G.nz;
     const sp<kG, const T> gco = coefs<T>(G);
     const sp<kS, T> lco = (sp<kS, T>)(base + tl.lds_co[d]);
-    for (int e = tid; e < nco; e += nthr) lco[e] = gco[e];
+    for (int e = ln; e < nco; e += 64) lco[e] = gco[e];
     if (d + 1 < nl) {
       const sp<kG, const double> gv = gp(G.vol);
       const sp<kS, double> lv = (sp<kS, double>)(base + tl.lds_vol[d]);
-      for (int e = tid; e < nv; e += nthr) lv[e] = gv[e];
+      for (int e = ln; e < nv; e += 64) lv[e] = gv[e];
     }
+  }
+  if (tl.lds_minv >= 0) {
+    const DevLevel& G = Lt[tl.lev[nl - 1]];
+    const int n2 = G.ncell * G.ncell;
+    const sp<kG, const double> gm = gp(G.minvT);
+    const sp<kS, T> lm = (sp<kS, T>)(base + tl.lds_minv);
+    for (int e = tid; e < n2; e += nthr) lm[e] = (T)gm[e];
   }
   const DevLevel& G0 = Lt[tl.lev[0]];
   {
@@ -851,7 +896,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
     if (type == kStepHalf) {
       if (tl.zline) {
         const int n = L.hx * L.ny;
-        for (int e = tid; e < n; e += nthr) zline_column<T, NZ>(L, U(d), Fv(d), CO(d), par, zg, e);
+        for (int e = tid; e < n; e += nthr) zline_column_n<T, NZ>(L, U(d), Fv(d), CO(d), par, zg, e);
       } else {
         const int n = L.hx * L.ny * L.nz;
         for (int e = tid; e < n; e += nthr) rb_point_elem<T>(L, U(d), Fv(d), CO(d), e, par, zg);
@@ -864,7 +909,12 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
         else restrict_any<T>(L, U(d), Fv(d), CO(d), VOL(d), C, Fv(d + 1), x, e);
       }
     } else if (type == kStepCoarse) {
-      for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, U(d), Fv(d), r);
+      if (tl.lds_minv >= 0) {
+        const sp<kS, const T> M = (sp<kS, const T>)(base + tl.lds_minv);
+        for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, U(d), Fv(d), M, r);
+      } else {
+        for (int r = tid; r < L.ncell; r += nthr) coarse_row<T>(L, U(d), Fv(d), gp(L.minvT), r);
+      }
     } else if (type == kStepProlong) {
       const DevXfer x = X[tl.xf[d]];
       const DevLevel& C = Ls[d + 1];
@@ -934,8 +984,11 @@ __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restric
 enum { kResMax = 0, kDefect32 = 1, kCorrDefect32 = 2 };
 
 // The marching loop of one thread.  Every field pointer is a __restrict__
-// parameter, so the compiler may issue the next planes' loads ahead of this
-// plane's stores (the outputs never alias the inputs).
+// parameter (the outputs never alias the inputs), and the loop is
+// software-pipelined by hand: the loads of plane k+1 are issued before plane
+// k is computed and stored, so a thread keeps two planes of loads in flight.
+// In mixed mode the centre column's p and e are carried separately (e for
+// the correction norm).
 template <int MODE, int NV>
 __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
                                            sp<kG, const float> __restrict__ ev,
@@ -947,41 +1000,70 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
                                            double wxp, double wym, double wyp, double dxy_y,
                                            double dxy_x, int k0, int k1, double (&v)[NV]) {
   constexpr bool MIXED = MODE == kCorrDefect32;
-  auto val = [&](int idx) -> double {
-    if constexpr (MIXED) return u[idx] + (double)ev[idx];
-    else return u[idx];
+  // loads of plane k: the four in-plane neighbours, b, and the centre at k+1
+  struct Plane {
+    double xm, xp, ym, yp, fb, pu;
+    float xme, xpe, yme, ype, pe;
+  };
+  auto load = [&](int k) {
+    Plane q;
+    const int pb = k * plane;
+    const int kp = (k == nz - 1) ? 0 : k + 1;
+    q.xm = u[pb + oxm]; q.xp = u[pb + oxp]; q.ym = u[pb + oym]; q.yp = u[pb + oyp];
+    q.fb = f[pb + col];
+    q.pu = u[kp * plane + col];
+    if constexpr (MIXED) {
+      q.xme = ev[pb + oxm]; q.xpe = ev[pb + oxp]; q.yme = ev[pb + oym]; q.ype = ev[pb + oyp];
+      q.pe = ev[kp * plane + col];
+    } else {
+      q.xme = q.xpe = q.yme = q.ype = q.pe = 0.0f;
+    }
+    return q;
+  };
+  auto sum = [&](double a, float b) -> double {
+    if constexpr (MIXED) return a + (double)b;
+    else return a;
   };
   const int km0 = (k0 == 0) ? nz - 1 : k0 - 1;
-  double vm = val(km0 * plane + col);
-  double vc = val(k0 * plane + col);
-#pragma unroll 4
-  for (int k = k0; k < k1; ++k) {
-    const int kp = (k == nz - 1) ? 0 : k + 1;
+  double vm = sum(u[km0 * plane + col], MIXED ? ev[km0 * plane + col] : 0.0f);
+  const float ec0 = MIXED ? ev[k0 * plane + col] : 0.0f;
+  double vc = sum(u[k0 * plane + col], ec0);
+  float ec = ec0;
+  auto step = [&](int k, const Plane& cur) {
     const int e = k * plane + col;
-    const double vp = val(kp * plane + col);
-    const int pb = k * plane;
-    const double xm = val(pb + oxm), xp = val(pb + oxp), ym = val(pb + oym), yp = val(pb + oyp);
-    double s = wxm * xm + wxp * xp;
-    s += wym * ym;
-    s += wyp * yp;
+    const double vp = sum(cur.pu, cur.pe);
+    double s = wxm * sum(cur.xm, cur.xme) + wxp * sum(cur.xp, cur.xpe);
+    s += wym * sum(cur.ym, cur.yme);
+    s += wyp * sum(cur.yp, cur.ype);
     s += cz[k] * vm;
     s += cz[nz + k] * vp;
     const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
-    const double fb = f[e];
-    const double res = fb - (s + diag * vc);
+    const double res = cur.fb - (s + diag * vc);
     if constexpr (MODE == kCorrDefect32) {
       pout[e] = vc;
       r32[e] = (float)res;
       v[0] = nmax(v[0], fabs(res));
-      v[1] = nmax(v[1], fabs((double)ev[e]));
+      v[1] = nmax(v[1], fabs((double)ec));
       v[2] = nmax(v[2], fabs(vc));
     } else {
       if constexpr (MODE == kDefect32) r32[e] = (float)res;
       v[0] = nmax(v[0], fabs(res));
-      v[1] = nmax(v[1], fabs(fb));
+      v[1] = nmax(v[1], fabs(cur.fb));
     }
     vm = vc;
     vc = vp;
+    ec = cur.pe;
+  };
+  // two plane buffers used alternately (no register copies), and the
+  // prefetch is unconditional (past the slab it reloads the last plane), so
+  // the wait before a plane's arithmetic covers that plane's loads only
+  Plane pa = load(k0), pb;
+  for (int k = k0; k < k1; k += 2) {
+    pb = load(min(k + 1, k1 - 1));
+    step(k, pa);
+    if (k + 1 >= k1) break;
+    pa = load(min(k + 2, k1 - 1));
+    step(k + 1, pb);
   }
 }
 
@@ -1325,7 +1407,8 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                   int segs, hipStream_t s) {
   const dim3 blk(64, segs);
-  const size_t lds = (size_t)(6 * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
+  // reduced-system state (two buffers in FP32, one in FP64) + staged rows
+  const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
   if (fp64)
     hipLaunchKernelGGL((k_zline<double, kZSeg, kZChunk64>), dim3(ll.nblocks), blk, lds, s, L, ll, parity,
                        zero_guess);

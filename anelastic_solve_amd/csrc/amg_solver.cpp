@@ -231,9 +231,13 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     const size_t limit = tail_lds_limit();
     auto al = [](size_t b) { return (b + 15) & ~(size_t)15; };
     // LDS of one tail level: u, f, coefficient table, cell widths
+    // the coarsest level's dense inverse is staged in LDS when it is small
+    const size_t nco = (size_t)I->H.lv[L - 1].ncell();
+    const size_t minv_bytes = nco * nco * w <= kTailMinvLds ? al(nco * nco * w) : 0;
     auto level_bytes = [&](const HostLevel& H, bool coarsest) {
       const size_t n1 = (size_t)H.nx + H.ny + H.nz;
-      return 2 * al((size_t)H.ncell() * w) + al(3 * n1 * w) + (coarsest ? 0 : al(n1 * 8));
+      return 2 * al((size_t)H.ncell() * w) + al(3 * n1 * w) +
+             (coarsest ? minv_bytes : al(n1 * 8));
     };
     int l0 = L - 1;
     size_t bytes = level_bytes(I->H.lv[L - 1], true);
@@ -273,6 +277,7 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
       tl.lds_co[d] = take(3 * n1 * w);
       tl.lds_vol[d] = (d + 1 < tl.nlev) ? take(n1 * 8) : 0;
     }
+    tl.lds_minv = minv_bytes ? take(minv_bytes) : -1;
     // the sub-V-cycle as a step list, in the order of mgMK/mg_cycle.F90
     // (pre-smoothing and restriction down, the coarsest solve, prolongation
     // and post-smoothing up); from a zero guess the first red pass sees
