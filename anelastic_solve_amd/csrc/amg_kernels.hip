@@ -838,6 +838,21 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
 // in LDS (the first level's f, and u when it is an input, are copied in,
 // its u copied out); the element routines are the standalone kernels' ones
 // instantiated with LDS pointers.  One workgroup per item (DevTail).
+// Global -> LDS copy of n values (converted to D) by `nthr` threads: each
+// thread issues its (up to) 8 loads before any store, so a copy costs one
+// memory round trip per 8*nthr values instead of one per nthr.
+template <class D, class SRC, class DST>
+__device__ __forceinline__ void stage_copy(DST dst, SRC src, int n, int t, int nthr) {
+  for (int e0 = t; e0 < n; e0 += 8 * nthr) {
+    D v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = (D)src[min(e0 + q * nthr, n - 1)];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (e0 + q * nthr < n) dst[e0 + q * nthr] = v[q];
+  }
+}
+
 template <class T, int NZ, bool ALL222>
 __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
                                                           const DevXfer* __restrict__ X,
@@ -860,32 +875,16 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
   for (int d = wv; d < nl; d += blockDim.y) {
     const DevLevel& G = Lt[tl.lev[d]];
     const int nco = 3 * (G.nx + G.ny + G.nz), nv = G.nx + G.ny + G.nz;
-    const sp<kG, const T> gco = coefs<T>(G);
-    const sp<kS, T> lco = (sp<kS, T>)(base + tl.lds_co[d]);
-    for (int e = ln; e < nco; e += 64) lco[e] = gco[e];
-    if (d + 1 < nl) {
-      const sp<kG, const double> gv = gp(G.vol);
-      const sp<kS, double> lv = (sp<kS, double>)(base + tl.lds_vol[d]);
-      for (int e = ln; e < nv; e += 64) lv[e] = gv[e];
-    }
+    stage_copy<T>((sp<kS, T>)(base + tl.lds_co[d]), coefs<T>(G), nco, ln, 64);
+    if (d + 1 < nl) stage_copy<double>((sp<kS, double>)(base + tl.lds_vol[d]), gp(G.vol), nv, ln, 64);
   }
   if (tl.lds_minv >= 0) {
     const DevLevel& G = Lt[tl.lev[nl - 1]];
-    const int n2 = G.ncell * G.ncell;
-    const sp<kG, const double> gm = gp(G.minvT);
-    const sp<kS, T> lm = (sp<kS, T>)(base + tl.lds_minv);
-    for (int e = tid; e < n2; e += nthr) lm[e] = (T)gm[e];
+    stage_copy<T>((sp<kS, T>)(base + tl.lds_minv), gp(G.minvT), G.ncell * G.ncell, tid, nthr);
   }
   const DevLevel& G0 = Lt[tl.lev[0]];
-  {
-    const sp<kG, const T> gf = gfield<const T>((const void*)G0.f);
-    const sp<kG, const T> gu = gfield<const T>((const void*)G0.u);
-    const sp<kS, T> lf = Fv(0), lu = U(0);
-    for (int e = tid; e < G0.ncell; e += nthr) {
-      lf[e] = gf[e];
-      if (tl.copy_u_in) lu[e] = gu[e];
-    }
-  }
+  stage_copy<T>(Fv(0), gfield<const T>((const void*)G0.f), G0.ncell, tid, nthr);
+  if (tl.copy_u_in) stage_copy<T>(U(0), gfield<const T>((const void*)G0.u), G0.ncell, tid, nthr);
   __syncthreads();
   // one routine per step type, so that each element routine is instantiated
   // once (the kernel stays small enough for the instruction cache)

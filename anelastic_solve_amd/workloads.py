@@ -104,3 +104,160 @@ def psource_host(U, V, W, grid, dt=1.0):
         + (0.5 * (rp + rk) * W[1:-1, 1:-1, 1:-1] - 0.5 * (rk + rm) * W[0:-2, 1:-1, 1:-1]) * rdzc
     ) * (1.0 / dt)
     return src
+
+
+# ---- stratified shear-layer projection sequence ---------------------------------
+# Workload generator for the sequence of pressure solves a time stepper
+# produces: an explicit centred advection (the workload's own, in torch on
+# the device; not part of the solver) advanced with the low-storage RK3 of
+# dns/NSsolver.F90:198-261, each substep followed by the projection
+# Psource -> mg_solver (warm start from the previous pressure) -> Pgrad, and
+# the FP64 divergence check of the projected field.
+
+RK3_STAGES = [  # (A_k, B_k, projection time step / delt) of dns/NSsolver.F90:198-261
+    (0.0, 1.0 / 3.0, 1.0 / 3.0),
+    (-5.0 / 9.0, 15.0 / 16.0, 5.0 / 12.0),
+    (-153.0 / 128.0, 8.0 / 15.0, 1.0 / 4.0),
+]
+
+
+def shear_layer(n=(512, 256, 256), beta=2.0, seed=2, Lx=4 * math.pi, Ly=2 * math.pi, delta=0.02,
+                amp=1e-3):
+    """Initial state of the shear layer: u = tanh((z-0.5)/delta) plus seeded
+    perturbations of amplitude ``amp`` in the Kelvin-Helmholtz band (x
+    wavelengths 4-16 shear thicknesses, Gaussian in z around the mid-plane),
+    density perturbation rho' = -0.5 tanh((z-0.5)/delta).  Returns
+    (grid, U, V, W, R) as ghosted host arrays (R at cell centres)."""
+    from .grid import Grid, tanh_axis, uniform_axis
+    nx, ny, nz = n
+    g = Grid(uniform_axis(nx, Lx), uniform_axis(ny, Ly), tanh_axis(nz, 1.0, beta))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    zc = g.az.xc[:, None, None]
+    xf = g.ax.xe[None, None, :]          # x faces (U)
+    xc = g.ax.xc[None, None, :]
+    yc = g.ay.xc[None, :, None]
+    env = np.exp(-((zc - 0.5) / (4 * delta)) ** 2)
+    lam = rng.uniform(4 * 2 * math.pi * delta, 16 * 2 * math.pi * delta, size=6)
+    kx = np.maximum(1, np.round(Lx / lam)) * 2 * math.pi / Lx
+    ph = rng.uniform(0, 2 * math.pi, size=(6, 2))
+    ky = rng.integers(0, 3, size=6) * 2 * math.pi / Ly
+    pert_u = np.zeros(g.shape)
+    pert_w = np.zeros(g.shape)
+    zf = g.az.xe[:, None, None]          # z faces (W)
+    envw = np.exp(-((zf - 0.5) / (4 * delta)) ** 2)
+    for q in range(6):
+        pert_u += np.cos(kx[q] * xf + ky[q] * yc + ph[q, 0]) * env
+        pert_w += np.sin(kx[q] * xc + ky[q] * yc + ph[q, 1]) * envw
+    pert_u *= amp / max(np.abs(pert_u).max(), 1e-300)
+    pert_w *= amp / max(np.abs(pert_w).max(), 1e-300)
+    U = np.tanh((zc - 0.5) / delta) + pert_u + np.zeros(g.shape)
+    V = amp * 0.1 * rng.standard_normal(g.shape) * env
+    W = pert_w.copy()
+    R = -0.5 * np.tanh((zc - 0.5) / delta) + np.zeros(g.shape)
+    U, V, W = finish_velocity(U, V, W)
+    return g, U, V, W, R
+
+
+class ShearSequence:
+    """Device state of the shear layer and its RK3 substeps with projection.
+
+    ``run(solver, steps)`` advances ``steps`` time steps (3 substeps each,
+    one pressure solve per substep, warm-started) and returns one record per
+    solve: V-cycles, relative residual, device milliseconds of the solve and
+    the post-projection divergence ratio max|D u| / max|D u*|.
+    """
+
+    def __init__(self, grid, U, V, W, R, device="cuda", cfl=0.2, gravity=1.0):
+        import torch
+        self.torch = torch
+        self.g = grid
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+        self.U, self.V, self.W, self.R = t(U), t(V), t(W), t(R)
+        self.p = torch.zeros_like(self.U)
+        self.src = torch.zeros_like(self.U)
+        self.q = [torch.zeros_like(self.U) for _ in range(4)]
+        self.gravity = gravity
+        dx = float(grid.ax.dxc[1])
+        self.dt = cfl * dx / max(1.0, float(self.U.abs().max()))
+        nz = grid.nz
+        self.rdx = 1.0 / float(grid.ax.dxc[1])
+        self.rdy = 1.0 / float(grid.ay.dxc[1])
+        zc = torch.from_numpy(grid.az.xc).to(device)
+        # centred z derivative at cell centres 1..nz from centres k-1, k+1
+        self.rdz2 = (1.0 / (zc[2:nz + 2] - zc[0:nz]))[:, None, None]
+        # linear interpolation weight of centre k+1 at face k (faces 1..nz-1)
+        ze = torch.from_numpy(grid.az.xe).to(device)
+        self.wz = ((ze[1:nz] - zc[1:nz]) / (zc[2:nz + 1] - zc[1:nz]))[:, None, None]
+
+    def _wrap(self, F):
+        F[:, :, 0] = F[:, :, -2]
+        F[:, :, -1] = F[:, :, 1]
+        F[:, 0, :] = F[:, -2, :]
+        F[:, -1, :] = F[:, 1, :]
+
+    def _tendencies(self):
+        """Centred advective tendencies at the faces (and of rho' at the
+        centres) with buoyancy on w."""
+        torch = self.torch
+        U, V, W, R = self.U, self.V, self.W, self.R
+        nz = self.g.nz
+        # cell-centre velocities (ghosted in x/y by the periodic wrap)
+        uc = 0.5 * (U[:, :, 1:] + U[:, :, :-1])[:, :, :]
+        uc = torch.nn.functional.pad(uc, (1, 0))
+        vc = torch.nn.functional.pad(0.5 * (V[:, 1:, :] + V[:, :-1, :]), (0, 0, 1, 0))
+        wc = torch.nn.functional.pad(0.5 * (W[1:, :, :] + W[:-1, :, :]), (0, 0, 0, 0, 1, 0))
+
+        def adv(phi):
+            i = slice(1, -1)
+            dx = (phi[i, i, 2:] - phi[i, i, :-2]) * (0.5 * self.rdx)
+            dy = (phi[i, 2:, i] - phi[i, :-2, i]) * (0.5 * self.rdy)
+            pz = phi.clone()
+            pz[0] = phi[1]
+            pz[nz + 1] = phi[nz]
+            dz = (pz[2:, i, i] - pz[:-2, i, i]) * self.rdz2
+            out = torch.zeros_like(phi)
+            out[i, i, i] = -(uc[i, i, i] * dx + vc[i, i, i] * dy + wc[i, i, i] * dz)
+            self._wrap(out)
+            return out
+
+        tu, tv, tw, tr = adv(uc), adv(vc), adv(wc), adv(R)
+        TU = torch.zeros_like(U)
+        TV = torch.zeros_like(V)
+        TW = torch.zeros_like(W)
+        TU[:, :, :-1] = 0.5 * (tu[:, :, :-1] + tu[:, :, 1:])
+        TV[:, :-1, :] = 0.5 * (tv[:, :-1, :] + tv[:, 1:, :])
+        b = tw - self.gravity * R
+        TW[1:nz] = (1.0 - self.wz) * b[1:nz] + self.wz * b[2:nz + 1]
+        for F in (TU, TV):
+            self._wrap(F)
+        self._wrap(TW)
+        TW[0] = 0.0
+        TW[nz] = 0.0
+        TW[nz + 1] = 0.0
+        return TU, TV, TW, tr
+
+    def run(self, solver, steps):
+        torch = self.torch
+        recs = []
+        for _ in range(steps):
+            for (a, b, f) in RK3_STAGES:
+                TU, TV, TW, TR = self._tendencies()
+                for qv, T, F in zip(self.q, (TU, TV, TW, TR), (self.U, self.V, self.W, self.R)):
+                    qv.mul_(a).add_(T, alpha=self.dt)
+                    F.add_(qv, alpha=b)
+                self._wrap(self.U); self._wrap(self.V); self._wrap(self.W); self._wrap(self.R)
+                nz = self.g.nz
+                self.W[0] = 0.0
+                self.W[nz] = 0.0
+                self.W[nz + 1] = 0.0
+                dt_rk = f * self.dt
+                solver.psource(self.U, self.V, self.W, self.src, dt_rk)
+                before = float(self.src.abs().max()) * dt_rk
+                it, relmax = solver.mg_solver(self.p, self.src)
+                st = solver.stats
+                solver.pgrad(self.p, self.U, self.V, self.W, dt_rk)
+                solver.psource(self.U, self.V, self.W, self.src, dt_rk)
+                after = float(self.src.abs().max()) * dt_rk
+                recs.append(dict(iters=it, relres=st.relres, ms=st.ms_total,
+                                 div_ratio=after / before if before > 0 else 0.0))
+        return recs

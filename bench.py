@@ -43,6 +43,15 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison")
     ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream section")
+    ap.add_argument("--no-shear", action="store_true", help="skip the shear-layer sequence")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0])
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--shear-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (the reference "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     return ap.parse_args()
 
 
@@ -90,9 +99,155 @@ def kernel_roofline(torch, solver, reps):
     ms = e0.elapsed_time(e1) / reps
     achieved = nbytes / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic("k_zline<float> finest"),
             "kernel": "k_zline<float> finest level, one colour (red half-sweep)",
             "bytes_per_launch": nbytes, "avg_launch_us": ms * 1e3}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel
+    calibration), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_stretched256.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        k = d["kernels"][kernel]
+        return k["fetch_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def bench_batch(torch, count=96, seed=3):
+    """Uneven batch (96 seeded instances): one launch per level step for all
+    of them vs one solve per launch, both to 1e-10 with FP64 correction."""
+    from anelastic_solve_amd import BatchSolver, PressureSolver, box_grid, default_config, workloads
+    specs = workloads.batch_specs(count, seed)
+    grids = [box_grid(*n, beta=b) for n, b in specs]
+    rhs = [torch.from_numpy(workloads.psource_host(*workloads.velocity_white(g, 100 + q), g)).cuda()
+           for q, g in enumerate(grids)]
+    cfg = default_config(precision="mixed", tol=1e-10, maxcy=100)
+    phis = [torch.zeros(g.shape, dtype=torch.float64, device="cuda") for g in grids]
+    bs = BatchSolver(grids, cfg)
+    bs.solve(phis, rhs)                       # warm-up (graphs, caches)
+    for p in phis:
+        p.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, rr = bs.solve(phis, rhs)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    bs.close()
+    solvers = [PressureSolver(g, cfg) for g in grids]
+    for q, s in enumerate(solvers):           # warm-up
+        phis[q].zero_()
+        s.mg_solver(phis[q], rhs[q])
+    for p in phis:
+        p.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q, s in enumerate(solvers):
+        s.mg_solver(phis[q], rhs[q])
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    cells = sum(g.nx * g.ny * g.nz for g in grids)
+    return {"instances": count, "cells_total": cells, "batched_solves_per_s": count / tb,
+            "one_per_launch_solves_per_s": count / ts, "speedup": ts / tb,
+            "max_relres": max(rr), "max_iters": max(its), "wall_s_batched": tb}
+
+
+def bench_serve(torch, rates, nreq, members=64, seed=4):
+    """Served stream: 64 members (sizes seeded from the five served sizes,
+    tanh-stretched), Poisson arrivals at each rate, mixed precision to
+    1e-10 warm-started from each member's resident pressure; solves/s and
+    p50/p99 submit-to-completion latency, beside the capacity of plain FP64
+    solves one at a time (mean service time over one member per size)."""
+    import numpy as np
+    from anelastic_solve_amd import PressureSolver, Server, box_grid, default_config, workloads
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = [workloads.SERVE_SIZES[i] for i in rng.integers(0, len(workloads.SERVE_SIZES), members)]
+    grids = [box_grid(*n, beta=2.0) for n in sizes]
+    base = {}
+    for n in set(sizes):   # one RHS pair per size: a field and a perturbed successor
+        g = box_grid(*n, beta=2.0)
+        U, V, W = workloads.velocity_spectral(g, 40 + workloads.SERVE_SIZES.index(n))
+        b = workloads.psource_host(U, V, W, g)
+        pert = workloads.psource_host(*workloads.velocity_white(g, 90), g)
+        a = torch.from_numpy(b).cuda()
+        base[n] = (a, a + 0.05 * torch.from_numpy(pert).cuda() * (a.abs().max() / max(abs(pert).max(), 1e-300)))
+    outs = [torch.zeros(g.shape, dtype=torch.float64, device="cuda") for g in grids]
+    cfg = default_config(precision="mixed", tol=1e-10, maxcy=100)
+    srv = Server(cfg, hbm_budget=200e9, max_slots=members, max_batch=members)
+    mids = [srv.add_member(g) for g in grids]
+    for q, m in enumerate(mids):          # bind every member once (cold start)
+        srv.submit(m, base[sizes[q]][0], outs[q])
+    srv.drain()
+    res = {"members": members, "sizes": sorted(set(map(tuple, sizes))), "rates": []}
+    parity = [0] * members
+    for rate in rates:
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+        who = rng.integers(0, members, size=nreq)
+        reqs = []
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            while time.perf_counter() - t0 < arr[i]:
+                pass
+            q = int(who[i])
+            parity[q] ^= 1
+            reqs.append(srv.submit(mids[q], base[sizes[q]][parity[q]], outs[q]))
+        srv.drain()
+        t1 = time.perf_counter()
+        lat, its, rr = [], [], []
+        for r in reqs:
+            st = srv.poll(r)
+            lat.append(st[3]); its.append(st[1]); rr.append(st[2])
+        res["rates"].append({"offered_req_per_s": rate, "requests": nreq,
+                             "solves_per_s": nreq / (t1 - t0),
+                             "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                             "mean_iters": float(np.mean(its)), "max_relres": max(rr)})
+    srv.close()
+    # plain FP64 one at a time: mean service time over the member mix
+    f64 = default_config(precision="fp64", tol=1e-10, maxcy=100)
+    svc = {}
+    for n in set(sizes):
+        s = PressureSolver(box_grid(*n, beta=2.0), f64)
+        p = torch.zeros(base[n][0].shape, dtype=torch.float64, device="cuda")
+        s.mg_solver(p, base[n][0])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.mg_solver(p, base[n][1])           # warm-started successor, as served
+        torch.cuda.synchronize()
+        svc[n] = time.perf_counter() - t0
+        s.close()
+    mean_svc = sum(svc[n] for n in sizes) / len(sizes)
+    res["fp64_one_at_a_time_capacity_req_per_s"] = 1.0 / mean_svc
+    return res
+
+
+def bench_shear(torch, steps):
+    """Shear-layer projection sequence at 512x256x256: 3 RK3 substeps per
+    step, each projected with a warm-started mixed solve to 1e-10 and
+    checked in FP64 (post-projection divergence relative to the pre-projection one)."""
+    from anelastic_solve_amd import PressureSolver, default_config, workloads
+    g, U, V, W, R = workloads.shear_layer((512, 256, 256))
+    s = PressureSolver(g, default_config(precision="mixed", tol=1e-10, maxcy=100))
+    seq = workloads.ShearSequence(g, U, V, W, R)
+    del U, V, W, R
+    seq.run(s, 1)                              # warm-up step (3 solves)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = seq.run(s, steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    s.close()
+    ms = [r["ms"] for r in recs]
+    return {"grid": [512, 256, 256], "solves": len(recs), "wall_s": wall,
+            "solve_ms_mean": sum(ms) / len(ms), "solve_ms_max": max(ms),
+            "mean_iters": sum(r["iters"] for r in recs) / len(recs),
+            "max_relres": max(r["relres"] for r in recs),
+            "max_post_projection_div_ratio": max(r["div_ratio"] for r in recs)}
 
 
 def cpu_baseline(n, beta, tol, cycles, iters_needed):
@@ -128,8 +283,22 @@ def cpu_baseline(n, beta, tol, cycles, iters_needed):
                        f"{n[0]}x{n[1]}x{n[2]}; {dt:.2f} s/cycle x {iters_needed} cycles per solve")}
 
 
+def copy_calibration():
+    import torch
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}), flush=True)
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note, not measured)"}))
@@ -213,6 +382,19 @@ def main():
         line["fp64"] = {k: f[k] for k in ("solves_per_s", "ms_per_solve", "iters", "relres",
                                          "vcycles_per_s", "cycle_GBps", "p50_ms", "p99_ms")}
         line["speedup_mixed_vs_fp64"] = m["solves_per_s"] / f["solves_per_s"]
+    torch.cuda.empty_cache()
+    if not a.no_batch:
+        line["batch"] = bench_batch(torch)
+        print("# batch done", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if not a.no_serve:
+        line["serve"] = bench_serve(torch, a.serve_rates, a.serve_requests)
+        print("# serve done", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+    if not a.no_shear:
+        line["shear"] = bench_shear(torch, a.shear_steps)
+        print("# shear done", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.beta, a.tol, a.cpu_cycles, m["iters"])
     print(json.dumps(line), flush=True)
