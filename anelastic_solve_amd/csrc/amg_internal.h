@@ -29,6 +29,7 @@ struct DevLevel {
   int inst;                  // instance id (batched/served runs)
   int ncell;                 // nx*ny*nz
   int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
+  int nullsp;                // the operator is singular (periodic/Neumann): constants in the null space
   int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
   double vsum;               // finest: sum of cell volumes
   double vrsum;              // finest: sum of V*rho
@@ -185,18 +186,24 @@ void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int
 // per item in 'ptr') and the FP64 finest level.
 // from_fortran: dst = (which==0 ? u : f) = src interior (+ bcfold if add_fold)
 // wsum:         {sum V x} (x = u or f), or {sum V rho} when which == 2
-// shift:        x -= (s[item] / (with_rho ? vrsum : vsum)) * (with_rho ? rho(z) : 1)
-//               with s the weighted sums of launch_wsum (after finalize)
-// to_fortran:   dst interior = u
+// to_fortran:   dst interior = u, or u - (c[item]/vrsum) rho(z) for singular
+//               items when c is given (c: the sums of launch_wsum)
+// ingest:       f = rhs interior + bcfold, u = phi interior, and one partial
+//               of sum V*f per block (kWsumPer cells per thread, as wsum)
+// first_defect: the first defect of a solve (march_blocks per item), as
+//               residual_max (fp64) / defect32 (mixed), after f -= c[item]/vsum
+//               in place for singular items (c: the sums of launch_ingest)
 // copy_u:       u2 = u
 void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr,
                          int which, int add_fold, hipStream_t s);
 void launch_wsum(const DevLevel* L, const LaunchList& ll, int which, double* partial,
                  hipStream_t s);
-void launch_shift(const DevLevel* L, const LaunchList& ll, int which, const double* c,
-                  int with_rho, hipStream_t s);
 void launch_to_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr,
-                       hipStream_t s);
+                       const double* c, hipStream_t s);
+void launch_ingest(const DevLevel* L, const LaunchList& ll, const PtrList& rhs, const PtrList& phi,
+                   double* partial, hipStream_t s);
+void launch_first_defect(bool fp64, const DevLevel* L, const LaunchList& ll, const double* c,
+                         double* partial, hipStream_t s);
 void launch_copy_u(const DevLevel* L, const LaunchList& ll, const int* active, hipStream_t s);
 
 // Ghost layers of a ghosted FP64 array (the serial path of mgMK/mg_bdry.F90).

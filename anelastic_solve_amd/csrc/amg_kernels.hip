@@ -988,10 +988,10 @@ enum { kResMax = 0, kDefect32 = 1, kCorrDefect32 = 2 };
 // k is computed and stored, so a thread keeps two planes of loads in flight.
 // In mixed mode the centre column's p and e are carried separately (e for
 // the correction norm).
-template <int MODE, int NV>
+template <int MODE, int NV, bool SHIFT>
 __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
                                            sp<kG, const float> __restrict__ ev,
-                                           sp<kG, const double> __restrict__ f,
+                                           sp<kG, double> __restrict__ f, double cs,
                                            sp<kG, double> __restrict__ pout,
                                            sp<kG, float> __restrict__ r32,
                                            sp<kG, const double> __restrict__ cz, int nz, int plane,
@@ -1009,7 +1009,7 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     const int pb = k * plane;
     const int kp = (k == nz - 1) ? 0 : k + 1;
     q.xm = u[pb + oxm]; q.xp = u[pb + oxp]; q.ym = u[pb + oym]; q.yp = u[pb + oyp];
-    q.fb = f[pb + col];
+    q.fb = f[pb + col] - cs;      // cs: the compatibility shift of b (first defect only)
     q.pu = u[kp * plane + col];
     if constexpr (MIXED) {
       q.xme = ev[pb + oxm]; q.xpe = ev[pb + oxp]; q.yme = ev[pb + oym]; q.ype = ev[pb + oyp];
@@ -1038,6 +1038,7 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     s += cz[nz + k] * vp;
     const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
     const double res = cur.fb - (s + diag * vc);
+    if constexpr (SHIFT) f[e] = cur.fb;
     if constexpr (MODE == kCorrDefect32) {
       pout[e] = vc;
       r32[e] = (float)res;
@@ -1066,8 +1067,9 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
   }
 }
 
-template <int MODE>
+template <int MODE, bool SHIFT>
 __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                 const double* __restrict__ csum,
                                                  double* __restrict__ partial) {
   constexpr int NV = MODE == kCorrDefect32 ? 3 : 2;
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -1093,8 +1095,10 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
     const auto co = gp(L.co64);
     const auto cy = co + 3 * nx;
     const auto cz = cy + 3 * ny;
-    fine_march<MODE, NV>(gfield<const double>((const void*)L.u), gfield<const float>((const void*)S.u),
-                         gfield<const double>((const void*)L.f), gfield<double>(L.u2),
+    const double cs = (SHIFT && L.nullsp) ? csum[t] / L.vsum : 0.0;
+    fine_march<MODE, NV, SHIFT>(gfield<const double>((const void*)L.u),
+                         gfield<const float>((const void*)S.u), gfield<double>(L.f), cs,
+                         gfield<double>(L.u2),
                          gfield<float>(S.f), cz, nz, plane, col, (j * 2 + oth) * hx + ml,
                          (j * 2 + oth) * hx + mr, (jm * 2 + oth) * hx + m, (jp * 2 + oth) * hx + m,
                          co[i], co[nx + i], cy[j], cy[ny + j], cy[2 * ny + j], co[2 * nx + i], k0, k1, v);
@@ -1111,12 +1115,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double
   const int t = blockIdx.x;
   const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int q = 0; q < nvals; ++q) {
-    double a = 0.0;
-    for (int b = b0 + threadIdx.x; b < b1; b += kBlock) {
-      const double x = partial[(size_t)b * nvals + q];
-      a = op_sum ? a + x : nmax(a, x);
+  // one pass over the partials for every value (nvals <= 4)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = b0 + threadIdx.x; b < b1; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < nvals) {
+        const double x = partial[(size_t)b * nvals + q];
+        acc[q] = op_sum ? acc[q] + x : nmax(acc[q], x);
+      }
     }
+  }
+  for (int q = 0; q < nvals; ++q) {
+    double a = acc[q];
     a = op_sum ? wave_sum(a) : wave_max(a);
     if (lane == 0) sm[w] = a;
     __syncthreads();
@@ -1153,6 +1164,38 @@ __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restr
   gfield<double>(which == 0 ? L.u : L.f)[e] = v;
 }
 
+// Ingest of a solve: b = rhs + boundary fold and u = phi (reference layout
+// -> chequer), with the partial sum of V*b of the block's cells (each thread
+// folds kWsumPer cells in a fixed order).
+__global__ __launch_bounds__(kBlock) void k_ingest(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                   PtrList rhs, PtrList phi,
+                                                   double* __restrict__ partial) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevLevel& L = Lt[ll.items[t]];
+  const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
+  const auto vol = gp(L.vol);
+  const auto bf = gp(L.bcfold);
+  const auto src = gp(rhs.p[t]);
+  const auto ph = gp(phi.p[t]);
+  const auto f = gfield<double>(L.f);
+  const auto u = gfield<double>(L.u);
+  double v[1] = {0.0};
+  for (int s = 0; s < kWsumPer; ++s) {
+    const int e = e0 + s * kBlock;
+    if (e < L.ncell) {
+      int i, j, k;
+      decode(L, e, i, j, k);
+      const size_t fi = fidx(i, j, k, L.nx, L.ny);
+      const double bv = src[fi] + ((bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k]);
+      f[e] = bv;
+      u[e] = ph[fi];
+      v[0] += (vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i]) * bv;
+    }
+  }
+  block_reduce_store<1, true>(v, partial + b);
+}
+
 // Weighted sums: each thread folds kWsumPer cells (in a fixed order), one
 // partial per block.
 __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt, LaunchList ll,
@@ -1182,26 +1225,11 @@ __global__ void k_zero(double* __restrict__ a, int n) {
   if (e < n) a[e] = 0.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_shift(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                  int which, const double* __restrict__ c,
-                                                  int with_rho) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevLevel& L = Lt[ll.items[t]];
-  const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
-  if (e >= L.ncell) return;
-  const auto x = gfield<double>(which == 0 ? L.u : L.f);
-  double s = c[t] / (with_rho ? L.vrsum : L.vsum);
-  if (with_rho) {
-    int i, j, k;
-    decode(L, e, i, j, k);
-    s *= gp(L.rho)[k];
-  }
-  x[e] -= s;
-}
-
+// dst interior = u, minus the null-space component (c[t]/vrsum) rho(z) of
+// singular items when c is given
 __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restrict__ Lt,
-                                                       LaunchList ll, PtrList ptr) {
+                                                       LaunchList ll, PtrList ptr,
+                                                       const double* __restrict__ c) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
@@ -1209,7 +1237,9 @@ __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restric
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
-  gp(ptr.p[t])[fidx(i, j, k, L.nx, L.ny)] = gfield<const double>((const void*)L.u)[e];
+  double x = gfield<const double>((const void*)L.u)[e];
+  if (c && L.nullsp) x -= (c[t] / L.vrsum) * gp(L.rho)[k];
+  gp(ptr.p[t])[fidx(i, j, k, L.nx, L.ny)] = x;
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ Lt, LaunchList ll,
@@ -1490,17 +1520,26 @@ size_t tail_lds_limit() {
   return limit;
 }
 void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH(k_fine<kResMax>, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH((k_fine<kResMax, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
+}
+void launch_first_defect(bool fp64, const DevLevel* L, const LaunchList& ll, const double* c,
+                         double* partial, hipStream_t s) {
+  if (fp64) LAUNCH((k_fine<kResMax, true>), ll.nblocks, kBlock, L, ll, c, partial);
+  else LAUNCH((k_fine<kDefect32, true>), ll.nblocks, kBlock, L, ll, c, partial);
+}
+void launch_ingest(const DevLevel* L, const LaunchList& ll, const PtrList& rhs, const PtrList& phi,
+                   double* partial, hipStream_t s) {
+  LAUNCH(k_ingest, ll.nblocks, kBlock, L, ll, rhs, phi, partial);
 }
 void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
   LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial);
 }
 void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH(k_fine<kDefect32>, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH((k_fine<kDefect32, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
 }
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
                              hipStream_t s) {
-  LAUNCH(k_fine<kCorrDefect32>, ll.nblocks, kBlock, L, ll, partial);
+  LAUNCH((k_fine<kCorrDefect32, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
 }
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
@@ -1516,12 +1555,9 @@ void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList&
 void launch_wsum(const DevLevel* L, const LaunchList& ll, int which, double* partial, hipStream_t s) {
   LAUNCH(k_wsum, ll.nblocks, kBlock, L, ll, which, partial);
 }
-void launch_shift(const DevLevel* L, const LaunchList& ll, int which, const double* c, int with_rho,
-                  hipStream_t s) {
-  LAUNCH(k_shift, ll.nblocks, kBlock, L, ll, which, c, with_rho);
-}
-void launch_to_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, hipStream_t s) {
-  LAUNCH(k_to_fortran, ll.nblocks, kBlock, L, ll, ptr);
+void launch_to_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, const double* c,
+                       hipStream_t s) {
+  LAUNCH(k_to_fortran, ll.nblocks, kBlock, L, ll, ptr, c);
 }
 void launch_copy_u(const DevLevel* L, const LaunchList& ll, const int* active, hipStream_t s) {
   LAUNCH(k_copy_u, ll.nblocks, kBlock, L, ll, active);

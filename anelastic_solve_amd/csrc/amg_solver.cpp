@@ -158,6 +158,7 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     d.bcz0 = I->H.bc[2][0]; d.bcz1 = I->H.bc[2][1];
     d.vsum = I->H.vol_sum;
     d.vrsum = I->H.vol_rho_sum;
+    d.nullsp = I->H.singular ? 1 : 0;
     const size_t nco = H.co.size();
     double* c64 = (double*)I->mem.alloc(nco * 8);
     float* c32 = (float*)I->mem.alloc(nco * 4);
@@ -585,23 +586,23 @@ void Context::post_norms(const std::vector<Instance*>& S) {
 // Singular instances: rhs == true removes the incompatible part of b
 // (b -= sum V b / sum V); otherwise removes the null-space part of the
 // current pressure (p -= (sum V p / sum V rho) rho).
-void Context::remove_mean(const std::vector<Instance*>& S, bool rhs) {
-  std::vector<Item> sums, shifts;
+// sum V p of the current iterate of every instance of S (item order of S), or
+// nullptr when no instance is singular
+const double* Context::null_sums(const std::vector<Instance*>& S) {
+  bool any = false;
+  std::vector<Item> sums;
   for (Instance* I : S) {
-    if (!I->H.singular) continue;
-    const int li = rhs ? I->fine[0] : I->fine[I->cur];
+    any = any || I->H.singular;
+    const int li = I->fine[I->cur];
     sums.push_back({li, cdiv(hL[li].ncell, (long long)kBlock * kWsumPer)});
-    shifts.push_back({li, cdiv(hL[li].ncell, kBlock)});
   }
-  if (sums.empty()) return;
+  if (!any) return nullptr;
   double* c = d_out + kOutRegion * 3;
   for_lists(sums, [&](const LaunchList& ll, int ioff, int boff) {
-    launch_wsum(dL, ll, rhs ? 1 : 0, d_partial + boff, stream);
+    launch_wsum(dL, ll, 0, d_partial + boff, stream);
     launch_finalize(ll, d_partial + boff, 1, 1, c + ioff, 1, stream);
   });
-  for_lists(shifts, [&](const LaunchList& ll, int ioff, int) {
-    launch_shift(dL, ll, rhs ? 1 : 0, c + ioff, rhs ? 0 : 1, stream);
-  });
+  return c;
 }
 
 int Context::prepare(const std::vector<Instance*>& S) {
@@ -615,26 +616,26 @@ int Context::prepare(const std::vector<Instance*>& S) {
     I->converged = false;
     I->done = false;
     I->status = AMG_OK;
-    items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, kBlock)});
+    items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, (long long)kBlock * kWsumPer)});
   }
-  // Fortran arrays -> chequer FP64 (b with the boundary fold)
-  for_lists(items, [&](const LaunchList& ll, int ioff, int) {
+  // Fortran arrays -> chequer FP64 (b with the boundary fold), and sum V b
+  double* c = d_out + kOutRegion * 3;
+  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     for (int t = 0; t < ll.nitems; ++t) {
       rhs.p[t] = const_cast<double*>(S[ioff + t]->d_rhs);
       phi.p[t] = S[ioff + t]->d_phi;
     }
-    launch_from_fortran(dL, ll, rhs, 1, 1, stream);
-    launch_from_fortran(dL, ll, phi, 0, 0, stream);
+    launch_ingest(dL, ll, rhs, phi, d_partial + boff, stream);
+    launch_finalize(ll, d_partial + boff, 1, 1, c + ioff, 1, stream);
   });
-  // singular: remove the incompatible part of b (sum V b = 0)
-  remove_mean(S, true);
-  // initial residual (mixed: also the FP32 defect of the first cycle)
+  // initial residual (mixed: also the FP32 defect of the first cycle); for a
+  // singular operator the incompatible part of b is removed first, in place
+  // (b -= sum V b / sum V)
   const bool fp64 = S[0]->fp64;
   const auto fitems = march_items(*this, S, false);
   for_lists(fitems, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
-    if (fp64) launch_residual_max(dL, ll, part, stream);
-    else launch_defect32(dL, ll, part, stream);
+    launch_first_defect(fp64, dL, ll, c + ioff, part, stream);
     launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
   });
   AMG_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream));
@@ -663,12 +664,12 @@ int Context::finish(const std::vector<Instance*>& S) {
     const int li = I->fine[I->cur];
     cur.push_back({li, cdiv(hL[li].ncell, kBlock)});
   }
-  // null-space removal: p -= (sum V p / sum V rho) rho
-  remove_mean(S, false);
+  // null-space removal, fused into the copy out: p -= (sum V p / sum V rho) rho
+  const double* c = null_sums(S);
   PtrList phi{};
   for_lists(cur, [&](const LaunchList& ll, int ioff, int) {
     for (int t = 0; t < ll.nitems; ++t) phi.p[t] = S[ioff + t]->d_phi;
-    launch_to_fortran(dL, ll, phi, stream);
+    launch_to_fortran(dL, ll, phi, c ? c + ioff : nullptr, stream);
   });
   // ghost layers as mg_bdry(ngrid, phif, .true., .true.) (mgMK/mg_solver.F90:139)
   for (Instance* I : S) {

@@ -102,7 +102,7 @@ struct Context {
   void vcycle(const std::vector<Instance*>& S, bool fp64);
   void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
   int prepare(const std::vector<Instance*>& S);
-  void remove_mean(const std::vector<Instance*>& S, bool rhs);
+  const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
   int cycle_and_norms(const std::vector<Instance*>& S);
   void post_norms(const std::vector<Instance*>& S);
