@@ -167,7 +167,10 @@ constexpr int kFinePer = 8;
 // The FP64 finest-level stencil kernels (residual, defect, correct+defect)
 // march in z over slabs of kMarchK planes: blocks per item =
 // cdiv(2*hx*ny, kBlock) * cdiv(nz, kMarchK), one partial per block.
-constexpr int kMarchK = 16;
+#ifndef AMG_MARCHK
+#define AMG_MARCHK 16
+#endif
+constexpr int kMarchK = AMG_MARCHK;
 inline long long march_blocks(int hx, int ny, int nz) {
   return ((2LL * hx * ny + kBlock - 1) / kBlock) * ((nz + kMarchK - 1) / kMarchK);
 }
