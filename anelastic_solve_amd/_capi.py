@@ -76,6 +76,14 @@ class amg_stats(C.Structure):
     ]
 
 
+class amg_flowbc(C.Structure):
+    """TB/VB(1:3,1:2,1:12) of the reference's module boundC, Fortran order."""
+    _fields_ = [
+        ("tb", C.c_int * 72),
+        ("vb", C.c_double * 72),
+    ]
+
+
 _PROTOS = {
     "amg_config_default": (None, [C.POINTER(amg_config)]),
     "amg_read_params": (C.c_int, [C.c_char_p, C.POINTER(amg_config)]),
@@ -93,6 +101,12 @@ _PROTOS = {
                                    C.c_double]),
     "amg_divergence_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]),
+    "amg_ghost_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.POINTER(amg_flowbc)]),
+    "amg_pgrad_ghost_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_double, C.POINTER(amg_flowbc)]),
+    "amg_cn_advance_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                        C.POINTER(amg_flowbc), C.c_double, C.c_double, C.c_double]),
     "amg_batch_create": (C.c_int, [C.c_int, C.POINTER(amg_config), C.POINTER(amg_grid),
                                    C.POINTER(C.c_void_p)]),
     "amg_batch_destroy": (C.c_int, [C.c_void_p]),

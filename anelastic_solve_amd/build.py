@@ -13,8 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libanelastic_mg.so")
 
-SOURCES = ["amg_kernels.hip", "amg_hierarchy.cpp", "amg_solver.cpp", "amg_server.cpp",
-           "amg_capi.cpp"]
+SOURCES = ["amg_kernels.hip", "amg_flow.hip", "amg_hierarchy.cpp", "amg_solver.cpp",
+           "amg_server.cpp", "amg_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -23,6 +23,8 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN = ["amg_cabi.F90", "amg_host_modules.F90", "amg_mg_shim.F90", "amg_shim_driver.F90"]
 FDIR = os.path.join(HERE, "fortran")
 FDRIVER = os.path.join(FDIR, "bin", "amg_shim_driver")
+# restart-dump reader/writer in Fortran (round-trip checks of restart.py)
+FRESTART = os.path.join(FDIR, "bin", "amg_restart_io")
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
@@ -86,6 +88,10 @@ def build_fortran(verbose=False):
     _run([FLANG, *objs, "-L" + HERE, "-lanelastic_mg", "-Wl,-rpath,$ORIGIN/../..", "-o", tmp],
          verbose)
     os.replace(tmp, FDRIVER)
+    tmp = FRESTART + ".tmp"
+    _run([FLANG, "-O2", "-module-dir", moddir, os.path.join(FDIR, "amg_restart_io.F90"), "-o", tmp],
+         verbose)
+    os.replace(tmp, FRESTART)
     return FDRIVER
 
 

@@ -13,6 +13,7 @@
 #include <string>
 
 #include "amg.h"
+#include "amg_flow.h"
 #include "amg_solver.h"
 
 using namespace amg;
@@ -24,6 +25,8 @@ struct amg_solver_s {
   double* d_rhs_stage = nullptr;
   double* d_div_partial = nullptr;
   double* h_div = nullptr;
+  CnRow* d_cn = nullptr;           // rows of the vertical CN system (nz+2)
+  CnRow* h_cn = nullptr;           //   and their pinned host staging
 };
 
 struct amg_batch_s {
@@ -35,6 +38,22 @@ namespace {
 int fail(int code, const std::string& msg) {
   set_error(msg);
   return code;
+}
+
+// the finest grid's ghost-inclusive metrics and rho(z), for the flow kernels
+FlowGrid flow_grid(const Instance* I) {
+  const HostLevel& F = I->H.lv[0];
+  FlowGrid G;
+  G.nx = F.nx;
+  G.ny = F.ny;
+  G.nz = F.nz;
+  const Axis* ax[3] = {&F.ax, &F.ay, &F.az};
+  for (int d = 0; d < 3; ++d) {
+    G.dxc[d] = ax[d]->dxc;
+    G.dxe[d] = ax[d]->dxe;
+  }
+  G.rho = F.rho;
+  return G;
 }
 
 // ---- INI reader with the reference's syntax (utils/readINI.F90) ------------
@@ -248,6 +267,8 @@ int amg_destroy(amg_handle h) {
   if (h->d_rhs_stage) (void)hipFree(h->d_rhs_stage);
   if (h->d_div_partial) (void)hipFree(h->d_div_partial);
   if (h->h_div) (void)hipHostFree(h->h_div);
+  if (h->d_cn) (void)hipFree(h->d_cn);
+  if (h->h_cn) (void)hipHostFree(h->h_cn);
   delete h;
   return AMG_OK;
 }
@@ -350,6 +371,76 @@ int amg_pgrad_device(amg_handle h, const double* p, double* U, double* V, double
   for (int d = 0; d < 3; ++d)
     if (bc[2 * d] == AMG_BC_PERIODIC)
       for (double* a : {U, V, W}) launch_ghost_wrap(a, nx, ny, nz, d, s);
+  AMG_HIP(hipGetLastError());
+  AMG_HIP(hipStreamSynchronize(s));
+  return AMG_OK;
+}
+
+int amg_ghost_device(amg_handle h, double* var, const char* vtype, const amg_flowbc* bc) {
+  if (!h || !var || !vtype || !bc) return fail(AMG_EINVAL, "amg_ghost_device: NULL argument");
+  const FlowGrid G = flow_grid(h->I);
+  GhostPlan P;
+  std::string err;
+  if (plan_ghost(G, vtype, *bc, &P, &err) != 0) return fail(AMG_EINVAL, "amg_ghost_device: " + err);
+  launch_flow_ghost(var, G.nx, G.ny, G.nz, P, h->ctx.stream);
+  AMG_HIP(hipGetLastError());
+  AMG_HIP(hipStreamSynchronize(h->ctx.stream));
+  return AMG_OK;
+}
+
+int amg_pgrad_ghost_device(amg_handle h, const double* p, double* U, double* V, double* W, double dt,
+                           const amg_flowbc* bc) {
+  if (!h || !p || !U || !V || !W || !bc) return fail(AMG_EINVAL, "amg_pgrad_ghost_device: NULL argument");
+  const FlowGrid G = flow_grid(h->I);
+  GhostPlan P[3];
+  std::string err;
+  const char* names[3] = {"utemp", "vtemp", "wtemp"};
+  for (int c = 0; c < 3; ++c)
+    if (plan_ghost(G, names[c], *bc, &P[c], &err) != 0)
+      return fail(AMG_EINVAL, "amg_pgrad_ghost_device: " + err);
+  const double *rho, *rdxc, *rdyc, *rdzc, *rdxe, *rdye, *rdze;
+  metric_ptrs(h->I, &rho, &rdxc, &rdyc, &rdzc, &rdxe, &rdye, &rdze);
+  const amg_config& cf = h->ctx.cfg;
+  const int wallmask = (cf.tbc[5] == AMG_BC_NEUMANN ? 1 : 0) | (cf.tbc[0] == AMG_BC_NEUMANN ? 2 : 0) |
+                       (cf.tbc[3] == AMG_BC_NEUMANN ? 4 : 0);
+  hipStream_t s = h->ctx.stream;
+  launch_pgrad(p, U, V, W, rho, rdxe, rdye, rdze, G.nx, G.ny, G.nz, dt, wallmask, s);
+  double* f[3] = {U, V, W};
+  for (int c = 0; c < 3; ++c) launch_flow_ghost(f[c], G.nx, G.ny, G.nz, P[c], s);
+  AMG_HIP(hipGetLastError());
+  AMG_HIP(hipStreamSynchronize(s));
+  return AMG_OK;
+}
+
+int amg_cn_advance_device(amg_handle h, const double* veltemp1, double* veltemp2, double f1,
+                          double f2, double f3, double rkdt, int var, const double* p,
+                          const amg_flowbc* bc, double rRe, double rPr, double rSc) {
+  if (!h || !veltemp1 || !veltemp2 || !bc) return fail(AMG_EINVAL, "amg_cn_advance_device: NULL argument");
+  if (veltemp1 == veltemp2) return fail(AMG_EINVAL, "amg_cn_advance_device: veltemp1 and veltemp2 alias");
+  static const char* ghost_type[9] = {nullptr, "u", "v", "w", nullptr, "rho", nullptr, nullptr, "scal1"};
+  if (var < 1 || var > 8 || !ghost_type[var])
+    return fail(AMG_EINVAL, "amg_cn_advance_device: ABORTING ADVANCE CN, VARID IS NOT VALID");
+  const FlowGrid G = flow_grid(h->I);
+  std::vector<CnRow> rows;
+  std::string err;
+  if (plan_cn(G, var, rkdt, rRe, rPr, rSc, *bc, &rows, &err) != 0)
+    return fail(AMG_EINVAL, "amg_cn_advance_device: " + err);
+  GhostPlan P;
+  if (plan_ghost(G, ghost_type[var], *bc, &P, &err) != 0)
+    return fail(AMG_EINVAL, "amg_cn_advance_device: " + err);
+  hipStream_t s = h->ctx.stream;
+  const size_t nb = sizeof(CnRow) * rows.size();
+  if (!h->d_cn) {
+    AMG_HIP(hipMalloc((void**)&h->d_cn, nb));
+    AMG_HIP(hipHostMalloc((void**)&h->h_cn, nb, hipHostMallocDefault));
+  }
+  std::memcpy(h->h_cn, rows.data(), nb);
+  AMG_HIP(hipMemcpyAsync(h->d_cn, h->h_cn, nb, hipMemcpyHostToDevice, s));
+  const double *rho, *rdxc, *rdyc, *rdzc, *rdxe, *rdye, *rdze;
+  metric_ptrs(h->I, &rho, &rdxc, &rdyc, &rdzc, &rdxe, &rdye, &rdze);
+  launch_cn_column(veltemp1, veltemp2, p, h->d_cn, rdxe, rdye, rdze, G.nx, G.ny, G.nz, f1, f2, f3,
+                   rkdt, var, s);
+  launch_flow_ghost(veltemp2, G.nx, G.ny, G.nz, P, s);
   AMG_HIP(hipGetLastError());
   AMG_HIP(hipStreamSynchronize(s));
   return AMG_OK;

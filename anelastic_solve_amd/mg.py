@@ -155,6 +155,41 @@ class PressureSolver:
                                          _dev_ptr(V, s, "V"), _dev_ptr(W, s, "W"), float(dt)),
               "Pgrad")
 
+    def pgrad_ghost(self, p, U, V, W, dt, bc):
+        """``Pgrad`` complete (``dns/NSsolver.F90:2023-2085``): the correction of
+        ``pgrad`` then ``ghost(U,'utemp')``, ``ghost(V,'vtemp')``,
+        ``ghost(W,'wtemp')`` with the flow BCs ``bc`` (a ``FlowBC``)."""
+        s = self.grid.shape
+        _sync_torch()
+        cbc = bc.to_c()
+        check(self._lib.amg_pgrad_ghost_device(self._h, _dev_ptr(p, s, "p"), _dev_ptr(U, s, "U"),
+                                               _dev_ptr(V, s, "V"), _dev_ptr(W, s, "W"), float(dt),
+                                               C.byref(cbc)), "Pgrad")
+
+    def ghost(self, var, vtype, bc):
+        """``ghost(var, vtype, stat)`` (``dns/BC.F90:1-187``) on a device field."""
+        s = self.grid.shape
+        _sync_torch()
+        cbc = bc.to_c()
+        check(self._lib.amg_ghost_device(self._h, _dev_ptr(var, s, "var"), vtype.encode(),
+                                         C.byref(cbc)), "ghost")
+
+    def var_advance_CN_PL(self, veltemp1, veltemp2, f1, f2, f3, rkdt, var, bc, rRe, rPr=1.0,
+                          rSc=1.0, p=None):
+        """``var_advance_CN_PL(veltemp1,veltemp2,f1,f2,f3,rkdt,var)``
+        (``dns/NSsolver.F90:3054-3465``): Crank-Nicolson in z of ``veltemp2``
+        (in place).  ``p`` given is the reference's ``pcorrect``; ``rRe``,
+        ``rPr``, ``rSc`` are 1/Re, 1/Pr, 1/Sc."""
+        s = self.grid.shape
+        _sync_torch()
+        cbc = bc.to_c()
+        pp = _dev_ptr(p, s, "p") if p is not None else C.c_void_p()
+        check(self._lib.amg_cn_advance_device(self._h, _dev_ptr(veltemp1, s, "veltemp1"),
+                                              _dev_ptr(veltemp2, s, "veltemp2"), float(f1), float(f2),
+                                              float(f3), float(rkdt), int(var), pp, C.byref(cbc),
+                                              float(rRe), float(rPr), float(rSc)),
+              "var_advance_CN_PL")
+
     def divergence(self, U, V, W, tempField=None):
         """Divergence report (``dns/Routines.F90:123-173``).
 
@@ -253,12 +288,26 @@ def Psource(solver, U, V, W, SRC, dt):
     return solver.psource(U, V, W, SRC, dt)
 
 
-def Pgrad(solver, p, U, V, W, dt):
-    return solver.pgrad(p, U, V, W, dt)
+def Pgrad(solver, p, U, V, W, dt, bc=None):
+    """``Pgrad(p,U,V,W,dt)``; with flow BCs ``bc`` also its three ghost calls."""
+    if bc is None:
+        return solver.pgrad(p, U, V, W, dt)
+    return solver.pgrad_ghost(p, U, V, W, dt, bc)
 
 
 def divergence(solver, U, V, W, tempField=None):
     return solver.divergence(U, V, W, tempField)
+
+
+def ghost(solver, var, vtype, bc):
+    """``ghost(var, vtype, stat)`` (``dns/BC.F90:1-187``)."""
+    return solver.ghost(var, vtype, bc)
+
+
+def var_advance_CN_PL(solver, veltemp1, veltemp2, f1, f2, f3, rkdt, var, bc, rRe, rPr=1.0, rSc=1.0,
+                      p=None):
+    """``var_advance_CN_PL`` (``dns/NSsolver.F90:3054-3465``)."""
+    return solver.var_advance_CN_PL(veltemp1, veltemp2, f1, f2, f3, rkdt, var, bc, rRe, rPr, rSc, p)
 
 
 # ---- batched and served solves ------------------------------------------------------

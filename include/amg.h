@@ -180,6 +180,45 @@ int amg_pgrad_device(amg_handle h, const double* p, double* U, double* V, double
 int amg_divergence_device(amg_handle h, const double* U, const double* V, const double* W,
                           double* div_or_null, double* avg, double* maxabs, int loc[3]);
 
+/* ---- the time stepper's boundary handling and vertical viscous step -------
+ * Boundary conditions of the flow fields, the reference's module boundC
+ * (include/modules.F90:82-85): TB(1:3,1:2,1:12) (type) and VB(1:3,1:2,1:12)
+ * (value) per direction, face and variable class, in Fortran order
+ * (index (d-1) + 3*(f-1) + 6*(v-1)), so a Fortran caller passes its arrays
+ * as they are.  Types: 1 Dirichlet, 2 Neumann, 3 periodic, 4 Neumann on the
+ * ghost only, 6 wall (mean = value, z), 7 wall (zero, z), 8 linear
+ * extrapolation (x). */
+typedef struct amg_flowbc {
+  int    tb[72];
+  double vb[72];
+} amg_flowbc;
+
+/* ghost(var, vtype, stat) (dns/BC.F90:1-187, bound_update :334-584), serial:
+ * every ghost takes its periodic image, then each face's rule runs in the
+ * order X1min, X1max, X2min, X2max, X3min, X3max.  vtype is the reference's
+ * type name ("u", "utemp", "p", "rho", "cfluc", ...); an unknown name is
+ * AMG_EINVAL (the reference prints and goes on). */
+int amg_ghost_device(amg_handle h, double* var, const char* vtype, const amg_flowbc* bc);
+
+/* Pgrad(p,U,V,W,dt) complete (dns/NSsolver.F90:2023-2085): the velocity
+ * correction of amg_pgrad_device followed by ghost(U,'utemp'),
+ * ghost(V,'vtemp'), ghost(W,'wtemp') with the given flow BCs. */
+int amg_pgrad_ghost_device(amg_handle h, const double* p, double* U, double* V, double* W,
+                           double dt, const amg_flowbc* bc);
+
+/* var_advance_CN_PL(veltemp1,veltemp2,f1,f2,f3,rkdt,var)
+ * (dns/NSsolver.F90:3054-3465), the DNS branch on one GPU: Crank-Nicolson in
+ * z for variable var (1 u, 2 v, 3 w, 5 rho, 8 scal1) of
+ *   veltemp2 <- f1*veltemp2 + f2*veltemp1*f3 + rkdt/2 (L veltemp2 + L veltemp2_new)
+ * with the wall rows of bc, a Thomas solve per interior column
+ * (mgMK/mg_relax.F90:541-595) and ghost(veltemp2, var's type).  p_or_null
+ * non-NULL is the reference's pcorrect (predicted pressure gradient).
+ * rRe, rPr, rSc are 1/Re, 1/Pr, 1/Sc (module Parameters).  Wall types the
+ * reference stops on, and systems with a zero pivot, are AMG_EINVAL. */
+int amg_cn_advance_device(amg_handle h, const double* veltemp1, double* veltemp2, double f1,
+                          double f2, double f3, double rkdt, int var, const double* p_or_null,
+                          const amg_flowbc* bc, double rRe, double rPr, double rSc);
+
 /* ---- batched uneven instances (one launch per level step) ---------------- */
 typedef struct amg_batch_s* amg_batch;
 
