@@ -438,8 +438,8 @@ int amg_cn_advance_device(amg_handle h, const double* veltemp1, double* veltemp2
   AMG_HIP(hipMemcpyAsync(h->d_cn, h->h_cn, nb, hipMemcpyHostToDevice, s));
   const double *rho, *rdxc, *rdyc, *rdzc, *rdxe, *rdye, *rdze;
   metric_ptrs(h->I, &rho, &rdxc, &rdyc, &rdzc, &rdxe, &rdye, &rdze);
-  launch_cn_column(veltemp1, veltemp2, p, h->d_cn, rdxe, rdye, rdze, G.nx, G.ny, G.nz, f1, f2, f3,
-                   rkdt, var, s);
+  launch_cn_column(veltemp1, veltemp2, p, h->d_cn, rdxe, rdye, G.nx, G.ny, G.nz, f1, f2, f3, rkdt, var,
+                   s);
   launch_flow_ghost(veltemp2, G.nx, G.ny, G.nz, P, s);
   AMG_HIP(hipGetLastError());
   AMG_HIP(hipStreamSynchronize(s));

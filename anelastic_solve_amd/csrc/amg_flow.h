@@ -43,6 +43,7 @@ struct CnRow {
   double ap, bp, c;
   double cval;
   double c1, m1, c2, m2, m3;
+  double pz;        // 1/dze(k): the predicted-pressure metric of w
   int kind;
 };
 
@@ -53,7 +54,7 @@ void launch_flow_ghost(double* a, int nx, int ny, int nz, const GhostPlan& P, hi
 int plan_cn(const FlowGrid& G, int var, double rkdt, double rRe, double rPr, double rSc,
             const amg_flowbc& bc, std::vector<CnRow>* rows, std::string* err);
 void launch_cn_column(const double* v1, double* v2, const double* p, const CnRow* T,
-                      const double* rdxe, const double* rdye, const double* rdze, int nx, int ny,
-                      int nz, double f1, double f2, double f3, double rkdt, int var, hipStream_t s);
+                      const double* rdxe, const double* rdye, int nx, int ny, int nz, double f1,
+                      double f2, double f3, double rkdt, int var, hipStream_t s);
 
 }  // namespace amg

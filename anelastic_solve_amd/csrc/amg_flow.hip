@@ -159,54 +159,114 @@ __global__ __launch_bounds__(kFlowBlock) void k_flow_face(double* __restrict__ a
 // G_k = r_k + ap_k G_{k-1}, written into v2 as it goes (the old v2 values
 // the next rows need are held in registers), and backward
 // x_k = (G_k - c_k x_{k+1}) / bp_k.  Table row k (CnRow) is wave-uniform.
+// Both passes run in chunks of kCnChunk rows whose loads (which do not
+// depend on the recurrence) are issued before the chunk's dependent steps,
+// so a thread keeps a chunk of loads in flight instead of one row.
+constexpr int kCnChunk = 8;
+
+// PV selects the predicted-pressure term at compile time: 0 none, 1 the x
+// neighbour (u), 2 the y neighbour (v), 3 the z neighbour (w), so the march
+// has no run-time branch on the variable.
+template <int PV>
 __global__ __launch_bounds__(kFlowBlock) void k_cn_column(const double* __restrict__ v1,
                                                           double* __restrict__ v2,
                                                           const double* __restrict__ p,
                                                           const CnRow* __restrict__ T,
                                                           const double* __restrict__ rdxe,
                                                           const double* __restrict__ rdye,
-                                                          const double* __restrict__ rdze,
                                                           int nx, int ny, int nz, double f1,
                                                           double f2, double f3, double rkdt,
-                                                          double h, int var) {
+                                                          double h) {
   const int t = blockIdx.x * kFlowBlock + threadIdx.x;
   if (t >= nx * ny) return;
   const int i = 1 + t % nx, j = 1 + t / nx;
   const size_t st = (size_t)(nx + 2) * (ny + 2);
   const size_t base = (size_t)j * (nx + 2) + i;
-  const bool pc = p != nullptr;
-  // old values of rows k-1, k (and k+1, loaded in the loop); row 0 is a
-  // constant row, so vm is first read at k = 1 (row 0)
+  const int n = nz + 2;
+  // the pressure neighbour is in the plane for u and v, one plane up for w
+  // (clamped at the top row, which is a constant row); its metric is
+  // per thread for u and v, per row (the table) for w
+  constexpr bool pcv = PV != 0;
+  const size_t poff = PV == 1 ? 1 : (size_t)(nx + 2);
+  auto pnb = [&](int k) -> size_t {
+    return PV == 3 ? base + (size_t)min(k + 1, n - 1) * st : base + (size_t)k * st + poff;
+  };
+  const double dmc = PV == 1 ? rdxe[i] : (PV == 2 ? rdye[j] : 0.0);
+  // old values of rows k-1 and k; row 0 is a constant row, so vm is first
+  // read at k = 1 (row 0)
   double vm = 0.0, vc = v2[base];
   double G = 0.0;
-  for (int k = 0; k <= nz + 1; ++k) {
-    const size_t e = base + (size_t)k * st;
+  // Every row evaluates the physics expression and a constant row then
+  // selects its value, so no load is used only under a branch (the compiler
+  // would move such loads down to their use and undo the prefetch).
+  auto step = [&](int k, double vp, double a1, double pa, double pb) {
     const CnRow R = T[k];
-    const double vp = k <= nz ? v2[e + st] : 0.0;
-    double r;
-    if (R.kind) {
-      r = R.cval;
-    } else {
-      r = f1 * vc + f2 * v1[e] * f3;
-      if (pc) {
-        if (var == 1) r = r - rkdt * (p[e + 1] - p[e]) * rdxe[i];
-        else if (var == 2) r = r - rkdt * (p[e + (nx + 2)] - p[e]) * rdye[j];
-        else if (var == 3) r = r - rkdt * (p[e + st] - p[e]) * rdze[k];
-      }
-      const double tv = R.c1 * (vp - vc) * R.m1 - R.c2 * (vc - vm) * R.m2;
-      r = r + h * tv * R.m3;
-    }
+    double r = f1 * vc + f2 * a1 * f3;
+    if constexpr (pcv) r = r - rkdt * (pb - pa) * (PV == 3 ? R.pz : dmc);
+    const double tv = R.c1 * (vp - vc) * R.m1 - R.c2 * (vc - vm) * R.m2;
+    r = r + h * tv * R.m3;
+    r = R.kind ? R.cval : r;
     G = k == 0 ? r : r + R.ap * G;
-    v2[e] = G;
+    v2[base + (size_t)k * st] = G;
     vm = vc;
     vc = vp;
+  };
+  // Forward, then backward, in self-contained chunks: all loads of a chunk
+  // are issued together, then its dependent steps run.  With one column
+  // per thread a 256^2 grid gives one wave per SIMD, so the chunks are made
+  // long (deep prefetch) rather than double-buffered: a buffer live across
+  // the loop's back edge would make the compiler wait for every load there.
+  constexpr int KF = PV ? 16 : 32;
+  constexpr int KB = 32;
+  const int ncf = n / KF;
+  for (int c = 0; c < ncf; ++c) {
+    double vp[KF], a1[KF], pa[KF], pb[KF];
+#pragma unroll
+    for (int q = 0; q < KF; ++q) {
+      const int kk = c * KF + q;
+      vp[q] = v2[base + (size_t)min(kk + 1, n - 1) * st];   // the top row's is unused
+      a1[q] = v1[base + (size_t)kk * st];
+      if constexpr (pcv) {
+        pa[q] = p[base + (size_t)kk * st];
+        pb[q] = p[pnb(kk)];
+      } else {
+        pa[q] = pb[q] = 0.0;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < KF; ++q) step(c * KF + q, vp[q], a1[q], pa[q], pb[q]);
   }
-  double x = G / T[nz + 1].bp;
-  v2[base + (size_t)(nz + 1) * st] = x;
-  for (int k = nz; k >= 0; --k) {
+  for (int k = ncf * KF; k < n; ++k) {
+    const double vp = v2[base + (size_t)min(k + 1, n - 1) * st];
+    const double a1 = v1[base + (size_t)k * st];
+    const double pa = pcv ? p[base + (size_t)k * st] : 0.0;
+    const double pb = pcv ? p[pnb(k)] : 0.0;
+    step(k, vp, a1, pa, pb);
+  }
+  // backward: the top row, the rows above the full chunks, then the chunks
+  // downward (chunk b holds rows rb - q with rb = n - 2 - r0 - b*KB)
+  double x = G / T[n - 1].bp;
+  v2[base + (size_t)(n - 1) * st] = x;
+  const int r0 = (n - 1) % KB;            // rows n-2 .. n-1-r0 go first, one by one
+  for (int k = n - 2; k > n - 2 - r0; --k) {
     const size_t e = base + (size_t)k * st;
     x = (v2[e] - T[k].c * x) / T[k].bp;
     v2[e] = x;
+  }
+  const int ncb = (n - 1) / KB;
+  for (int b = 0; b < ncb; ++b) {
+    const int rb = n - 2 - r0 - b * KB;
+    double g[KB];
+#pragma unroll
+    for (int q = 0; q < KB; ++q) g[q] = v2[base + (size_t)(rb - q) * st];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < KB; ++q) {
+      const int kk = rb - q;
+      x = (g[q] - T[kk].c * x) / T[kk].bp;
+      v2[base + (size_t)kk * st] = x;
+    }
   }
 }
 
@@ -329,6 +389,7 @@ int plan_cn(const FlowGrid& G, int var, double rkdt, double rRe, double rPr, dou
     CnRow& w = R[k];
     w.c1 = tmp2;
     w.c2 = tmp1;
+    w.pz = rdze[k];
     if (var == 3) {
       sub[k] = -0.5 * rkdt * tmp1 * rdze[k] * rdzc[k];
       sup[k] = -0.5 * rkdt * tmp2 * rdze[k] * rdzc[k + 1];
@@ -436,10 +497,15 @@ int plan_cn(const FlowGrid& G, int var, double rkdt, double rRe, double rPr, dou
 }
 
 void launch_cn_column(const double* v1, double* v2, const double* p, const CnRow* T,
-                      const double* rdxe, const double* rdye, const double* rdze, int nx, int ny,
-                      int nz, double f1, double f2, double f3, double rkdt, int var, hipStream_t s) {
-  hipLaunchKernelGGL(k_cn_column, dim3(cdiv((long long)nx * ny, kFlowBlock)), dim3(kFlowBlock), 0, s,
-                     v1, v2, p, T, rdxe, rdye, rdze, nx, ny, nz, f1, f2, f3, rkdt, rkdt * 0.5, var);
+                      const double* rdxe, const double* rdye, int nx, int ny, int nz, double f1,
+                      double f2, double f3, double rkdt, int var, hipStream_t s) {
+  const dim3 g(cdiv((long long)nx * ny, kFlowBlock)), b(kFlowBlock);
+  const int pv = (p != nullptr && var <= 3) ? var : 0;
+  const double h = rkdt * 0.5;
+  if (pv == 1) hipLaunchKernelGGL(k_cn_column<1>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
+  else if (pv == 2) hipLaunchKernelGGL(k_cn_column<2>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
+  else if (pv == 3) hipLaunchKernelGGL(k_cn_column<3>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
+  else hipLaunchKernelGGL(k_cn_column<0>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
 }
 
 }  // namespace amg

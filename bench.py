@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0])
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--shear-steps", type=int, default=20)
+    ap.add_argument("--no-flow", action="store_true",
+                    help="skip the time-stepper steps (CN, ghost, Pgrad with ghosts)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -250,6 +252,45 @@ def bench_shear(torch, steps):
             "max_post_projection_div_ratio": max(r["div_ratio"] for r in recs)}
 
 
+def bench_flow(torch, n, beta, reps=20):
+    """The time stepper's device steps at the workload grid (FP64, on the
+    solver's stream, HIP events around each entry point): the vertical
+    Crank-Nicolson step of u (var_advance_CN_PL, compulsory 24 B/cell: read
+    veltemp1 and veltemp2, write veltemp2), ghost of one field, and Pgrad
+    with its three ghost calls (compulsory 48 B/cell: read p, U, V, W,
+    write U, V, W -- p counted once)."""
+    from anelastic_solve_amd import FlowBC, PressureSolver, box_grid, default_config
+    g = box_grid(*n, beta=beta)
+    s = PressureSolver(g, default_config(precision="fp64"))
+    bc = FlowBC.make((True, True, False), {(3, 1, 1): (2, 0.0), (3, 2, 1): (2, 0.0),
+                                            (3, 1, 3): (1, 0.0), (3, 2, 3): (7, 0.0)})
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    f = [torch.randn(g.shape, dtype=torch.float64, device="cuda", generator=gen) for _ in range(4)]
+    st = torch.cuda.ExternalStream(s.stream)
+    N = float(n[0] * n[1] * n[2])
+
+    def timed(fn):
+        fn()
+        st.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    cn = timed(lambda: s.var_advance_CN_PL(f[0], f[1], 1.0, 0.5, 1.0, 1e-3, 1, bc, 1e-3))
+    gh = timed(lambda: s.ghost(f[1], "u", bc))
+    pg = timed(lambda: s.pgrad_ghost(f[0], f[1], f[2], f[3], 1e-3, bc))
+    s.close()
+    return {"grid": list(n), "cn_step_ms": cn, "cn_step_GBps": 24.0 * N / (cn * 1e-3) / 1e9,
+            "ghost_us": gh * 1e3, "pgrad_with_ghosts_ms": pg,
+            "pgrad_with_ghosts_GBps": 48.0 * N / (pg * 1e-3) / 1e9,
+            "note": "per call, including the host-side row table upload and the stream sync"}
+
+
 def cpu_baseline(n, beta, tol, cycles, iters_needed):
     """Bounded sample: `cycles` V-cycles of the numpy oracle (mixed precision,
     same grid and RHS) timed on the host; solves/s extrapolated with the GPU's
@@ -394,6 +435,9 @@ def main():
     if not a.no_shear:
         line["shear"] = bench_shear(torch, a.shear_steps)
         print("# shear done", file=sys.stderr, flush=True)
+    if not a.no_flow:
+        line["flow"] = bench_flow(torch, tuple(a.n), a.beta)
+        print("# flow done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.beta, a.tol, a.cpu_cycles, m["iters"])
