@@ -27,6 +27,8 @@ SMOOTHER_RBPGS, SMOOTHER_LJ = 0, 2
 FP64, MIXED = 0, 1
 STOP_RESIDUAL, STOP_RELMAX = 0, 1
 MAX_LEVELS = 16
+AMG_FP64, AMG_MIXED = 0, 1
+AMG_STOP_RESIDUAL, AMG_STOP_RELMAX = 0, 1
 
 _dp = C.POINTER(C.c_double)
 

@@ -48,11 +48,28 @@ Axis axis_from_arrays(int n, const double* xe, const double* xc, const double* d
   return a;
 }
 
-Axis coarsen_axis(const Axis& a, int factor) {
+Axis coarsen_axis(const Axis& a, int factor, bool wall_lo, bool wall_hi) {
   if (factor == 1) return a;
   std::vector<double> F;
   for (int i = 0; i <= a.n; i += 2) F.push_back(a.xe[i]);
-  return axis_from_faces(F);
+  Axis c = axis_from_faces(F);
+  // At a wall the ghost centre stays where the finest level has it: a
+  // Dirichlet value sits at that point, and a coarse correction must vanish
+  // at the same place.  A coarse ghost half a coarse cell out would move the
+  // wall outward at every coarsening; with several z-only coarsenings the
+  // coarse-grid correction then no longer matches the fine problem and the
+  // V-cycle diverges.  (Neumann walls carry no flux, so the distance is
+  // unused there.)
+  const int n = c.n;
+  if (wall_lo) {
+    c.xc[0] = a.xc[0];
+    c.dxe[0] = c.xc[1] - c.xc[0];
+  }
+  if (wall_hi) {
+    c.xc[n + 1] = a.xc[a.n + 1];
+    c.dxe[n] = c.xc[n + 1] - c.xc[n];
+  }
+  return c;
 }
 
 namespace {
@@ -299,9 +316,11 @@ int build_hierarchy(const amg_config& cfg, const amg_grid& g, Hierarchy& h, std:
     const HostLevel& F = h.lv.back();
     HostLevel C;
     C.nx = F.nx / fac[t][0]; C.ny = F.ny / fac[t][1]; C.nz = F.nz / fac[t][2];
-    C.ax = coarsen_axis(F.ax, fac[t][0]);
-    C.ay = coarsen_axis(F.ay, fac[t][1]);
-    C.az = coarsen_axis(F.az, fac[t][2]);
+    const bool wall[3] = {h.bc[0][0] != AMG_BC_PERIODIC, h.bc[1][0] != AMG_BC_PERIODIC,
+                          h.bc[2][0] != AMG_BC_PERIODIC};
+    C.ax = coarsen_axis(F.ax, fac[t][0], wall[0], wall[0]);
+    C.ay = coarsen_axis(F.ay, fac[t][1], wall[1], wall[1]);
+    C.az = coarsen_axis(F.az, fac[t][2], wall[2], wall[2]);
     C.rho = coarsen_rho(F.rho, F.az, h.bc[2][0] == AMG_BC_PERIODIC, fac[t][2]);
     h.lv.back().fx = fac[t][0]; h.lv.back().fy = fac[t][1]; h.lv.back().fz = fac[t][2];
     h.lv.push_back(C);

@@ -19,7 +19,7 @@ struct Axis {
 Axis axis_from_faces(const std::vector<double>& faces);
 Axis axis_from_arrays(int n, const double* xe, const double* xc, const double* dxe,
                       const double* dxc);
-Axis coarsen_axis(const Axis& a, int factor);
+Axis coarsen_axis(const Axis& a, int factor, bool wall_lo, bool wall_hi);
 
 struct HostLevel {
   int nx = 0, ny = 0, nz = 0;

@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0])
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--shear-steps", type=int, default=20)
+    ap.add_argument("--no-production", action="store_true",
+                    help="skip the anelastic production configuration")
     ap.add_argument("--no-flow", action="store_true",
                     help="skip the time-stepper steps (CN, ghost, Pgrad with ghosts)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -291,6 +293,37 @@ def bench_flow(torch, n, beta, reps=20):
             "note": "per call, including the host-side row table upload and the stream sync"}
 
 
+def bench_production(torch, steps=5):
+    """The anelastic stratified-layer production configuration: 256 x 64 x 256
+    uniform cells (10000 x 2500 x 5000 m), rho(z) of set_densityBG, Dirichlet
+    pressure on the z walls, 4/3/8 levels, RBPGS with the run's per-level
+    sweeps (configs/anelastic_sl.ini).  Solves to a relative residual of
+    1e-10 (mixed and FP64), and with the run's own stop rule (relmax 1e-4)."""
+    from anelastic_solve_amd import PressureSolver, workloads
+    from anelastic_solve_amd._capi import AMG_STOP_RESIDUAL
+    g, cfg, bc, U, V, W = workloads.anelastic_sl()
+    rhs = torch.from_numpy(workloads.psource_host(U, V, W, g)).cuda()
+    out = {"grid": [g.nx, g.ny, g.nz], "levels": None}
+    s = PressureSolver(g, cfg)
+    times, _, _ = run_solves(torch, s, rhs, steps, 1)
+    ms = 1e3 * percentile(times, 50)
+    out["run_stop_rule"] = {"rule": "relmax <= 1e-4", "ms_per_solve": ms, "iters": s.stats.iters,
+                            "relmax": s.stats.relmax}
+    out["levels"] = [list(l) for l in s.levels]
+    s.close()
+    for prec in ("mixed", "fp64"):
+        cfg.stop_rule = AMG_STOP_RESIDUAL
+        cfg.tol = 1e-10
+        cfg.precision = 1 if prec == "mixed" else 0
+        s = PressureSolver(g, cfg)
+        times, _, _ = run_solves(torch, s, rhs, steps, 1)
+        ms = 1e3 * percentile(times, 50)
+        out[prec] = {"solves_per_s": 1e3 / ms, "ms_per_solve": ms, "iters": s.stats.iters,
+                     "relres": s.stats.relres}
+        s.close()
+    return out
+
+
 def cpu_baseline(n, beta, tol, cycles, iters_needed):
     """Bounded sample: `cycles` V-cycles of the numpy oracle (mixed precision,
     same grid and RHS) timed on the host; solves/s extrapolated with the GPU's
@@ -438,6 +471,9 @@ def main():
     if not a.no_flow:
         line["flow"] = bench_flow(torch, tuple(a.n), a.beta)
         print("# flow done", file=sys.stderr, flush=True)
+    if not a.no_production:
+        line["production"] = bench_production(torch)
+        print("# production done", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(n, a.beta, a.tol, a.cpu_cycles, m["iters"])

@@ -67,20 +67,33 @@ def tanh_axis(n, L, beta, x0=0.0):
     return axis_from_faces(x0 + L * z)
 
 
-def coarsen_axis(ax, factor=2):
+def coarsen_axis(ax, factor=2, wall=False):
     """Coarse axis by 2:1 agglomeration of cells (every other face).
 
     Replaces ``mgMK/mg_grids.F90:112-198``, whose periodic coarse spacing
     divides a ghost-inclusive length (``:64-66,121``); here the coarse faces
     are exactly the even fine faces, so a periodic uniform axis coarsens to
     exactly ``2 dx``.
+    At a wall (``wall``) the coarse ghost centres stay at the finest level's:
+    a Dirichlet value sits at that point, and the coarse corrections must
+    vanish at the same place.  A ghost half a coarse cell out would move the
+    wall at every coarsening, and after several z-only coarsenings the
+    coarse-grid correction no longer matches the fine problem (the V-cycle
+    diverges).  Neumann walls carry no flux, so the distance is unused there.
     """
     if factor == 1:
         return ax
     n = ax["n"]
     assert n % 2 == 0
     faces = ax["xe"][0:n + 1:2]
-    return axis_from_faces(faces)
+    c = axis_from_faces(faces)
+    if wall:
+        m = c["n"]
+        c["xc"][0] = ax["xc"][0]
+        c["dxe"][0] = c["xc"][1] - c["xc"][0]
+        c["xc"][m + 1] = ax["xc"][n + 1]
+        c["dxe"][m] = c["xc"][m + 1] - c["xc"][m]
+    return c
 
 
 def axis_coefs(ax, bc_lo, bc_hi):
@@ -216,8 +229,8 @@ def hierarchy(ax, ay, az, rho, tbc, factors):
                        "bc": (bx, by, bz)})
         if li < len(factors):
             fx, fy, fz = factors[li]
-            cur = (coarsen_axis(a, fx), coarsen_axis(b, fy), coarsen_axis(c, fz),
-                   coarsen_rho(r, c, bz[0] == PERIODIC, fz))
+            cur = (coarsen_axis(a, fx, bx[0] != PERIODIC), coarsen_axis(b, fy, by[0] != PERIODIC),
+                   coarsen_axis(c, fz, bz[0] != PERIODIC), coarsen_rho(r, c, bz[0] == PERIODIC, fz))
     return levels
 
 
