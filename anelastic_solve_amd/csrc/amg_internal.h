@@ -30,6 +30,8 @@ struct DevLevel {
   int ncell;                 // nx*ny*nz
   int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
   int nullsp;                // the operator is singular (periodic/Neumann): constants in the null space
+  unsigned mhx, mny;         // ceil(2^32/hx), ceil(2^32/ny) (0 for a divisor of 1): e/hx = umulhi(e, mhx)
+                             // exactly while e*hx < 2^32 (the coarse tail's small levels)
   int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
   double vsum;               // finest: sum of cell volumes
   double vrsum;              // finest: sum of V*rho
@@ -90,6 +92,7 @@ constexpr int kTailCells = 32768;
 constexpr int kTailNz = 32;      // z-line tail levels: whole columns in registers (FP32)
 constexpr int kTailNz64 = 16;    // the same for FP64 (register budget)
 constexpr int kTailWaves = 8;
+constexpr int kTailWavesPoint = 16;   // the point-smoother tail (no line-solve registers)
 constexpr int kTailMaxLevels = 16;
 // Steps of the tail's sub-V-cycle, precomputed by the host in cycle order:
 // code = type | level << 3 | parity << 7 | zero_guess << 8.
@@ -148,7 +151,7 @@ void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int pa
 // a launch may request on this device.
 // variant: the kernel's column length class (tail_nz_class) and whether
 // every tail transfer coarsens 2:1 in all three directions.
-int tail_nz_class(bool fp64, int nz_max);
+int tail_nz_class(bool fp64, int nz_max, bool zline);
 void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
                  const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s);
 size_t tail_lds_limit();

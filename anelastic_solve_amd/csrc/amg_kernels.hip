@@ -175,10 +175,8 @@ __device__ __forceinline__ T diag_at(CP co, const DevLevel& L, int i, int j, int
 // e enumerates the cells of the colour; in plane k the colour is column
 // colour (parity+k)&1, so consecutive e are one contiguous half-row.
 template <class T, class UP, class FP, class CP>
-__device__ __forceinline__ void rb_point_elem(const DevLevel& L, UP u, FP f, CP co, int e,
-                                              int parity, int zero_guess) {
-  const int hx = L.hx, ny = L.ny;
-  const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
+__device__ __forceinline__ void rb_point_mjk(const DevLevel& L, UP u, FP f, CP co, int m, int j,
+                                             int k, int parity, int zero_guess) {
   const int c = (k + parity) & 1;
   const int i = 2 * m + ((c + j) & 1);
   const Nb n = neighbours(L, i, j, k);
@@ -191,6 +189,27 @@ __device__ __forceinline__ void rb_point_elem(const DevLevel& L, UP u, FP f, CP 
     val = (f[n.c] - s) / d;
   }
   u[n.c] = val;
+}
+
+template <class T, class UP, class FP, class CP>
+__device__ __forceinline__ void rb_point_elem(const DevLevel& L, UP u, FP f, CP co, int e,
+                                              int parity, int zero_guess) {
+  const int hx = L.hx, ny = L.ny;
+  const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
+  rb_point_mjk<T>(L, u, f, co, m, j, k, parity, zero_guess);
+}
+
+// The same with the element index decoded by multiply-high reciprocals
+// (DevLevel::mhx, mny), exact for the small levels of the coarse tail.
+__device__ __forceinline__ int fast_div(int e, unsigned mag) {
+  return mag ? (int)__umulhi((unsigned)e, mag) : e;
+}
+template <class T, class UP, class FP, class CP>
+__device__ __forceinline__ void rb_point_small(const DevLevel& L, UP u, FP f, CP co, int e,
+                                               int parity, int zero_guess) {
+  const int row = fast_div(e, L.mhx), m = e - row * L.hx;
+  const int k = fast_div(row, L.mny), j = row - k * L.ny;
+  rb_point_mjk<T>(L, u, f, co, m, j, k, parity, zero_guess);
 }
 
 template <class T>
@@ -869,7 +888,9 @@ __device__ __forceinline__ void stage_copy(DST dst, SRC src, int n, int t, int n
 }
 
 template <class T, int NZ, bool ALL222>
-__global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
+// NZ = 0 is the point-smoother variant: no line-solve code (whose column
+// registers would cap the workgroup at 512 lanes), and 1024 lanes.
+__global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
                                                           const DevXfer* __restrict__ X,
                                                           const DevTail* __restrict__ Tt,
                                                           LaunchList ll) {
@@ -908,12 +929,14 @@ __global__ __launch_bounds__(64 * kTailWaves) void k_tail(const DevLevel* __rest
     const int type = code & 7, d = (code >> 3) & 15, par = (code >> 7) & 1, zg = (code >> 8) & 3;
     const DevLevel& L = Ls[d];
     if (type == kStepHalf) {
-      if (tl.zline) {
-        const int n = L.hx * L.ny;
-        for (int e = tid; e < n; e += nthr) zline_column_n<T, NZ>(L, U(d), Fv(d), CO(d), par, zg, e);
+      if (NZ > 0 && tl.zline) {
+        if constexpr (NZ > 0) {
+          const int n = L.hx * L.ny;
+          for (int e = tid; e < n; e += nthr) zline_column_n<T, NZ>(L, U(d), Fv(d), CO(d), par, zg, e);
+        }
       } else {
         const int n = L.hx * L.ny * L.nz;
-        for (int e = tid; e < n; e += nthr) rb_point_elem<T>(L, U(d), Fv(d), CO(d), e, par, zg);
+        for (int e = tid; e < n; e += nthr) rb_point_small<T>(L, U(d), Fv(d), CO(d), e, par, zg);
       }
     } else if (type == kStepRestrict) {
       const DevXfer x = X[tl.xf[d]];
@@ -1503,7 +1526,8 @@ void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream
   if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll);
   else LAUNCH(k_coarse<float>, ll.nblocks, kBlock, L, ll);
 }
-int tail_nz_class(bool fp64, int nz_max) {
+int tail_nz_class(bool fp64, int nz_max, bool zline) {
+  if (!zline) return 0;
   if (nz_max <= 8) return 8;
   if (nz_max <= 16 || fp64) return 16;
   return 32;
@@ -1512,16 +1536,20 @@ int tail_nz_class(bool fp64, int nz_max) {
 #define TAIL_KERNEL(T, NZ, F) (k_tail<T, NZ, F>)
 void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
                  const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s) {
-  const dim3 blk(64, kTailWaves), grd(ll.nitems);
+  const dim3 grd(ll.nitems);
 #define GO(TT, NZ)                                                                         \
   do {                                                                                     \
+    const dim3 blk(64, NZ == 0 ? kTailWavesPoint : kTailWaves);                            \
     if (all222) hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, true), grd, blk, lds_bytes, s, L, X, T, ll);  \
     else hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, false), grd, blk, lds_bytes, s, L, X, T, ll);        \
   } while (0)
   if (fp64) {
-    if (nz_class <= 8) GO(double, 8); else GO(double, 16);
+    if (nz_class == 0) GO(double, 0); else if (nz_class <= 8) GO(double, 8); else GO(double, 16);
   } else {
-    if (nz_class <= 8) GO(float, 8); else if (nz_class <= 16) GO(float, 16); else GO(float, 32);
+    if (nz_class == 0) GO(float, 0);
+    else if (nz_class <= 8) GO(float, 8);
+    else if (nz_class <= 16) GO(float, 16);
+    else GO(float, 32);
   }
 #undef GO
 }
@@ -1538,6 +1566,10 @@ size_t tail_lds_limit() {
   // room for the static descriptor array of the kernel
   lim -= sizeof(DevLevel) * kTailMaxLevels + 256;
   const void* kernels[] = {
+      reinterpret_cast<const void*>(&k_tail<double, 0, true>),
+      reinterpret_cast<const void*>(&k_tail<double, 0, false>),
+      reinterpret_cast<const void*>(&k_tail<float, 0, true>),
+      reinterpret_cast<const void*>(&k_tail<float, 0, false>),
       reinterpret_cast<const void*>(&k_tail<double, 8, true>),
       reinterpret_cast<const void*>(&k_tail<double, 8, false>),
       reinterpret_cast<const void*>(&k_tail<double, 16, true>),

@@ -147,6 +147,11 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     DevLevel& d = ents[l];
     std::memset(&d, 0, sizeof d);
     d.nx = H.nx; d.ny = H.ny; d.nz = H.nz; d.hx = H.nx / 2;
+    auto magic = [](unsigned v) -> unsigned {
+      return v <= 1 ? 0u : (unsigned)((0x100000000ull + v - 1) / v);
+    };
+    d.mhx = magic((unsigned)d.hx);
+    d.mny = magic((unsigned)d.ny);
     d.px = I->H.bc[0][0] == AMG_BC_PERIODIC;
     d.py = I->H.bc[1][0] == AMG_BC_PERIODIC;
     d.pz = I->H.bc[2][0] == AMG_BC_PERIODIC;
@@ -321,7 +326,7 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
       nzmax = std::max(nzmax, H.nz);
       if (d + 1 < tl.nlev && (H.fx != 2 || H.fy != 2 || H.fz != 2)) all222 = false;
     }
-    I->tail_nzc = tail_nz_class(I->fp64, nzmax);
+    I->tail_nzc = tail_nz_class(I->fp64, nzmax, cfg.smoother == AMG_SMOOTHER_LJ);
     I->tail_222 = all222;
     I->tail = (int)hT.size();
     I->tail_lds = off;
@@ -450,7 +455,7 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
     });
   }
   // the coarse tails, one launch per kernel variant
-  for (int nzc : {8, 16, 32}) {
+  for (int nzc : {0, 8, 16, 32}) {
     for (int a222 = 0; a222 < 2; ++a222) {
       std::vector<Item> items;
       size_t lds = 0;
