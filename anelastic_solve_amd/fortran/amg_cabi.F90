@@ -50,6 +50,13 @@ module amg_cabi
      integer(c_int) :: level_n(3, AMG_MAX_LEVELS)
   end type amg_stats
 
+  ! TB(1:3,1:2,1:12) and VB(1:3,1:2,1:12) of module boundC, flattened in
+  ! Fortran order (amg_flowbc_from copies them in).
+  type, bind(C) :: amg_flowbc
+     integer(c_int) :: tb(72)
+     real(c_double) :: vb(72)
+  end type amg_flowbc
+
   interface
      subroutine amg_config_default(cfg) bind(C, name="amg_config_default")
        import :: amg_config
@@ -112,6 +119,32 @@ module amg_cabi
        integer(c_int), intent(out) :: loc(3)
      end function amg_divergence_device
 
+     integer(c_int) function amg_ghost_device(handle, var, vtype, bc) &
+          bind(C, name="amg_ghost_device")
+       import :: c_int, c_ptr, c_char, amg_flowbc
+       type(c_ptr), value :: handle, var
+       character(kind=c_char), intent(in) :: vtype(*)
+       type(amg_flowbc), intent(in) :: bc
+     end function amg_ghost_device
+
+     integer(c_int) function amg_pgrad_ghost_device(handle, p, u, v, w, dt, bc) &
+          bind(C, name="amg_pgrad_ghost_device")
+       import :: c_int, c_ptr, c_double, amg_flowbc
+       type(c_ptr), value :: handle, p, u, v, w
+       real(c_double), value :: dt
+       type(amg_flowbc), intent(in) :: bc
+     end function amg_pgrad_ghost_device
+
+     integer(c_int) function amg_cn_advance_device(handle, veltemp1, veltemp2, f1, f2, f3, rkdt, &
+          var, p, bc, rRe, rPr, rSc) bind(C, name="amg_cn_advance_device")
+       import :: c_int, c_ptr, c_double, amg_flowbc
+       type(c_ptr), value :: handle, veltemp1, veltemp2, p
+       real(c_double), value :: f1, f2, f3, rkdt
+       integer(c_int), value :: var
+       type(amg_flowbc), intent(in) :: bc
+       real(c_double), value :: rRe, rPr, rSc
+     end function amg_cn_advance_device
+
      type(c_ptr) function amg_strerror(code) bind(C, name="amg_strerror")
        import :: c_ptr, c_int
        integer(c_int), value :: code
@@ -143,6 +176,15 @@ contains
        out(i:i) = chars(i)
     end do
   end subroutine amg_c_string
+
+  ! The flow BC tables of module boundC as the C struct amg_flowbc.
+  function amg_flowbc_from(TB, VB) result(bc)
+    integer, intent(in) :: TB(3, 2, 12)
+    real(c_double), intent(in) :: VB(3, 2, 12)
+    type(amg_flowbc) :: bc
+    bc%tb = int(reshape(TB, [72]), c_int)
+    bc%vb = reshape(VB, [72])
+  end function amg_flowbc_from
 
   ! "<strerror(code)>: <last error message>" for diagnostics.
   function amg_message(code) result(msg)

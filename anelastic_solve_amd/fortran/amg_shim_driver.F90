@@ -47,7 +47,22 @@ contains
     type(amg_config) :: c
     type(amg_grid) :: g
     type(amg_stats) :: s
-    print '(a,3(1x,i0))', 'LAYOUT', c_sizeof(c), c_sizeof(g), c_sizeof(s)
+    type(amg_flowbc) :: b
+    integer :: TB(3, 2, 12)
+    real(c_double) :: VB(3, 2, 12)
+    integer :: d, f, v
+    print '(a,4(1x,i0))', 'LAYOUT', c_sizeof(c), c_sizeof(g), c_sizeof(s), c_sizeof(b)
+    ! TB(d,f,v) = 100*d + 10*f + v lands at flat index (d-1) + 3*(f-1) + 6*(v-1)
+    do v = 1, 12
+       do f = 1, 2
+          do d = 1, 3
+             TB(d, f, v) = 100 * d + 10 * f + v
+             VB(d, f, v) = -TB(d, f, v)
+          end do
+       end do
+    end do
+    b = amg_flowbc_from(TB, VB)
+    print '(a,3(1x,i0),1x,f8.1)', 'FLOWBC', b%tb(1), b%tb(2 + 3 * 1 + 6 * 4), b%tb(72), b%vb(72)
   end subroutine layout
 
   subroutine print_params(st)

@@ -11,6 +11,7 @@ import ctypes
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from anelastic_solve_amd import _capi
@@ -31,9 +32,19 @@ def _run(*args, cwd=None):
 def test_bind_c_layouts_match_c():
     r = _run("layout")
     assert r.returncode == 0, r.stderr
-    sizes = [int(x) for x in r.stdout.split()[1:]]
-    assert sizes == [ctypes.sizeof(_capi.amg_config), ctypes.sizeof(_capi.amg_grid),
-                     ctypes.sizeof(_capi.amg_stats)]
+    rows = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.splitlines() if ln.strip()}
+    assert [int(x) for x in rows["LAYOUT"]] == [
+        ctypes.sizeof(_capi.amg_config), ctypes.sizeof(_capi.amg_grid),
+        ctypes.sizeof(_capi.amg_stats), ctypes.sizeof(_capi.amg_flowbc)]
+    # TB(2,2,5) and TB(3,2,12) at the flat indexes the C side reads
+    # ((d-1) + 3*(f-1) + 6*(v-1)), and the values beside them
+    from anelastic_solve_amd.flowbc import FlowBC
+    TB = np.zeros((3, 2, 12), dtype=np.int32)
+    for d, f, v in np.ndindex(3, 2, 12):
+        TB[d, f, v] = 100 * (d + 1) + 10 * (f + 1) + v + 1
+    c = FlowBC(TB, -TB.astype(float)).to_c()
+    assert [int(x) for x in rows["FLOWBC"][:3]] == [c.tb[0], c.tb[1 + 3 * 1 + 6 * 4], c.tb[71]] == [111, 225, 332]
+    assert float(rows["FLOWBC"][3]) == c.vb[71]
 
 
 def test_read_params_through_fortran(tmp_path):
