@@ -159,10 +159,10 @@ __global__ __launch_bounds__(kFlowBlock) void k_flow_face(double* __restrict__ a
 // G_k = r_k + ap_k G_{k-1}, written into v2 as it goes (the old v2 values
 // the next rows need are held in registers), and backward
 // x_k = (G_k - c_k x_{k+1}) / bp_k.  Table row k (CnRow) is wave-uniform.
-// Both passes run in chunks of kCnChunk rows whose loads (which do not
-// depend on the recurrence) are issued before the chunk's dependent steps,
-// so a thread keeps a chunk of loads in flight instead of one row.
-constexpr int kCnChunk = 8;
+// Both passes run in chunks of rows (16 or 32 forward, 32 backward) whose
+// loads (which do not depend on the recurrence) are issued before the
+// chunk's dependent steps, so a thread keeps a chunk of loads in flight
+// instead of one row.
 
 // PV selects the predicted-pressure term at compile time: 0 none, 1 the x
 // neighbour (u), 2 the y neighbour (v), 3 the z neighbour (w), so the march

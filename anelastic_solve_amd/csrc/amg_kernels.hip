@@ -229,15 +229,32 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
 // the columns (i,j) of one colour ((i+j)&1 == parity) solve their
 // tridiagonal z-system with the other colour's current values on the
 // right-hand side.  A workgroup takes 64 columns (one per lane) and splits
-// each column into P segments of S cells (wave p owns segment p): every
-// segment eliminates its rows forward (LU) and backward (UL) to express its
-// end values through the unknown end values of its neighbours, the 2P
-// interface unknowns are solved by one wave in LDS, and every segment then
-// solves its rows with the interface values known.  The result equals a
-// Thomas solve of the whole column up to rounding.  Periodic z: the wrap
-// couplings are lagged (the other end's old value on the right-hand side),
-// in the order of the oracle's restatement.
-template <class T, int S, int C>
+// each column into P segments of S cells (wave p owns segment p).  Every
+// segment eliminates its rows once, forward, for three right-hand sides:
+// its data, and unit couplings to the last value of the segment below and
+// to the first value of the segment above.  Its end values are then linear
+// in those two unknowns; the 2P interface unknowns are solved by parallel
+// cyclic reduction in LDS, and one back substitution per segment gives its
+// values.  The result equals a Thomas solve of the whole column up to
+// rounding.  Periodic z: the wrap couplings are lagged (the other end's old
+// value on the right-hand side), in the order of the oracle's restatement.
+// A row's coupling past either end of the column (a wrap or a wall ghost)
+// only ever multiplies an interface value that does not exist, which the
+// reduction takes as zero, so the rows are used as the tables hold them.
+
+// Row access at a uniform row base plus a per-lane 32-bit byte offset: the
+// global load/store form with a 64-bit scalar base and a 32-bit vector
+// offset, so a row's address costs no vector arithmetic.
+template <class T> __device__ __forceinline__ T ld_row(sp<kG, const T> row, unsigned boff) {
+  return *(sp<kG, const T>)((sp<kG, const char>)row + boff);
+}
+template <class T> __device__ __forceinline__ void st_row(sp<kG, T> row, unsigned boff, T v) {
+  *(sp<kG, T>)((sp<kG, char>)row + boff) = v;
+}
+
+// NBR false: the other colour is zero (zero_guess bit 0, the first
+// half-sweep from u = 0) and is not read.
+template <class T, int S, int C, bool NBR>
 __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int zero_guess, int col0,
                                             int p, int lane, int P, T* __restrict__ lds, bool act) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
@@ -258,10 +275,13 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   const sp<kG, T> u = gfield<T>(L.u);
   const sp<kG, const T> f = gfield<const T>((const void*)L.f);
   const int plane = ny * 2 * hx;
-  const int own = cidx(m, j, 0, c, hx, ny);
-  const int xo = cidx(0, j, 0, oth, hx, ny);
-  const int so = cidx(m, jm, 0, oth, hx, ny);
-  const int no = cidx(m, jp, 0, oth, hx, ny);
+  // byte offsets within a plane (the same in every row)
+  constexpr unsigned W = sizeof(T);
+  const unsigned bown = (unsigned)cidx(m, j, 0, c, hx, ny) * W;
+  const unsigned xo = (unsigned)cidx(0, j, 0, oth, hx, ny);
+  const unsigned bxl = (xo + ml) * W, bxr = (xo + mr) * W;
+  const unsigned bso = (unsigned)cidx(m, jm, 0, oth, hx, ny) * W;
+  const unsigned bno = (unsigned)cidx(m, jp, 0, oth, hx, ny) * W;
   const bool pz = L.pz != 0;
   const int Pi = (nz + S - 1) / S;            // segments of this item (<= blockDim.y)
   const int ks = p * S;
@@ -269,12 +289,11 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   // A (sub-diagonal), E (super-diagonal) and the z part of the diagonal
   // depend on k only and p is wave-uniform, so they are scalar values
   // (uniform loads issued with the rest, read without memory latency in the
-  // serial elimination chains); the lane's x/y diagonal share dxy is added
-  // at use.  The lagged periodic wrap couplings are zero in the matrix.
-  // All loads -- these rows, the lagged wrap values and the right-hand-side
-  // data -- are unconditional (every address is in bounds, uses are
-  // selected) and issued before the first wait: a sweep pays one memory
-  // round trip.
+  // serial elimination chain); the lane's x/y diagonal share dxy is added
+  // at use.  All loads -- these rows, the lagged wrap values and the
+  // right-hand-side data -- are unconditional (every address is in bounds,
+  // uses are selected) and issued before the first wait: a sweep pays one
+  // memory round trip.
   // FP64 rows do not fit in scalar registers: they are staged in LDS by
   // lanes 0..S-1 and read back as broadcasts (LDS_COEF).
   constexpr bool LDS_COEF = sizeof(T) == 8;
@@ -286,28 +305,26 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   T za = T(0), ze = T(0), zd = T(0);
   if constexpr (LDS_COEF) {
     const int kq = min(ks + (lane & (S - 1)), nz - 1);
-    const T a0 = cz[kq], e0 = cz[nz + kq];
-    za = (pz && kq == 0) ? T(0) : a0;
-    ze = (pz && kq == nz - 1) ? T(0) : e0;
+    za = cz[kq];
+    ze = cz[nz + kq];
     zd = cz[2 * nz + kq];
   } else {
+    // rows kc..kc+S-1 from one base (three 16-word scalar loads); the tables
+    // end with kZSeg spare words, so a ragged or spare segment stays in bounds
+    const sp<kG, const T> cr = cz + min(ks, nz - 1);
 #pragma unroll
     for (int q = 0; q < S; ++q) {
-      const int k = min(ks + q, nz - 1);
-      const T a0 = cz[k], e0 = cz[nz + k];
-      A[q] = (pz && k == 0) ? T(0) : a0;
-      E[q] = (pz && k == nz - 1) ? T(0) : e0;
-      Dz[q] = cz[2 * nz + k];
+      A[q] = cr[q];
+      E[q] = cr[nz + q];
+      Dz[q] = cr[2 * nz + q];
     }
   }
-  const T cz_first = cz[0], cz_last = cz[2 * nz - 1];
-  // zero_guess bit 0: the other colour is zero (first half-sweep from u = 0);
-  // bit 1: this colour's old values are zero (no lagged wrap coupling)
-  const bool nbr = !(zero_guess & 1);
+  // zero_guess bit 1: this colour's old values are zero (no lagged wrap)
   const bool lag = pz && !(zero_guess & 2);
-  const T ubot = u[(nz - 1) * plane + own], utop = u[own];
+  const T cz_first = cz[0], cz_last = cz[2 * nz - 1];
+  const T ubot = ld_row<T>(u + (nz - 1) * plane, bown), utop = ld_row<T>(u, bown);
   const T lagb = (lag && p == 0) ? cz_first * ubot : T(0);
-  const T lagt = (lag && ks + len == nz && len > 0) ? cz_last * utop : T(0);
+  const T lagt = (lag && ks + len == nz) ? cz_last * utop : T(0);
   T g[S];
 #pragma unroll
   for (int q0 = 0; q0 < S; q0 += C) {
@@ -317,11 +334,14 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
 #pragma unroll
     for (int qq = 0; qq < C; ++qq) {
       const int kb = min(ks + q0 + qq, nz - 1) * plane;   // rows clamped: in bounds
-      fv[qq] = f[kb + own];
-      xl[qq] = u[kb + xo + ml];
-      xr[qq] = u[kb + xo + mr];
-      ys[qq] = u[kb + so];
-      yn[qq] = u[kb + no];
+      fv[qq] = ld_row<T>(f + kb, bown);
+      if constexpr (NBR) {
+        const sp<kG, const T> ur = u + kb;
+        xl[qq] = ld_row<T>(ur, bxl);
+        xr[qq] = ld_row<T>(ur, bxr);
+        ys[qq] = ld_row<T>(ur, bso);
+        yn[qq] = ld_row<T>(ur, bno);
+      }
     }
     if (LDS_COEF && q0 == 0) {
       if (lane < S) {
@@ -333,84 +353,84 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     }
 #pragma unroll
     for (int qq = 0; qq < C; ++qq) {
-      const int q = q0 + qq;
-      T sn = wxl * xl[qq] + wxr * xr[qq];
-      sn += wys * ys[qq];
-      sn += wyn * yn[qq];
-      T v = nbr ? fv[qq] - sn : fv[qq];
-      v -= (ks + q == 0) ? lagb : T(0);
-      v -= (ks + q == nz - 1) ? lagt : T(0);
-      g[q] = (q < len) ? v : T(0);
+      if constexpr (NBR) {
+        T sn = wxl * xl[qq] + wxr * xr[qq];
+        sn += wys * ys[qq];
+        sn += wyn * yn[qq];
+        g[q0 + qq] = fv[qq] - sn;
+      } else {
+        g[q0 + qq] = fv[qq];
+      }
     }
   }
+  g[0] -= lagb;                                            // row 0 of the column (p == 0)
 #define A(q) (LDS_COEF ? cA[q] : A[LDS_COEF ? 0 : (q)])
 #define E(q) (LDS_COEF ? cE[q] : E[LDS_COEF ? 0 : (q)])
 #define D(q) ((LDS_COEF ? cD[q] : Dz[LDS_COEF ? 0 : (q)]) + dxy)
   const T a_first = A(0);                                  // coupling to u_{ks-1}
-  T c_last = T(0);                                         // coupling to u_{ks+len}
-#pragma unroll
-  for (int q = 0; q < S; ++q) c_last = (q == len - 1) ? E(q) : c_last;
-  // phase 1: interface data of segment p
-  // The eliminations are written for a segment of n rows; a full segment
-  // (n == S, every segment but the last of a ragged column) instantiates
-  // them with a constant n, so the unrolled chains carry no branches.
-  T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
-  auto interface_data = [&](const int len) {
-    // LU sweep: yl, vl, wl, w0
-    T cp = T(0), yf = T(0), vf = T(0), prod = T(1);
+  // phase 1: forward elimination of the segment's rows for the data (g, in
+  // place), the coupling a_first to u_{ks-1} entering row 0 (v) and the
+  // coupling E(len-1) to u_{ks+len} entering the last row; cp holds the
+  // eliminated super-diagonal, and cp[len-1] the last row's coupling (w).
+  // A full segment (n == S, every segment but the last of a ragged column)
+  // instantiates the passes with a constant n, so the unrolled chains carry
+  // no branches.
+  T vv[S], cp[S];
+  auto factor = [&](const int len) {
+    T cq = T(0), y = T(0), v = T(0);
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       if (q < len) {
+        if (q == len - 1) g[q] -= lagt;                    // row nz-1 (last segment)
         const T a = (q == 0) ? T(0) : A(q);
-        const T rd = rcp(D(q) - a * cp);
-        yf = (g[q] - a * yf) * rd;
-        vf = (q == 0) ? a_first * rd : -a * vf * rd;
-        if (q == len - 1) {
-          wl = c_last * rd;
-        } else {
-          cp = E(q) * rd;
-          prod *= -cp;
-        }
+        const T rd = rcp(D(q) - a * cq);
+        y = (g[q] - a * y) * rd;
+        v = (q == 0) ? a_first * rd : -(a * rd) * v;
+        cq = E(q) * rd;
+        g[q] = y;
+        vv[q] = v;
+        cp[q] = cq;
       }
     }
-    yl = yf;
-    vl = vf;
-    w0 = wl * prod;
-    // UL sweep: y0, v0
-    T ri = T(0), h = T(0);
-#pragma unroll
-    for (int q = S - 1; q >= 0; --q) {
-      if (q < len) {
-        if (q < len - 1) {
-          const T ri2 = rcp(D(q) - E(q) * A(q + 1) * ri);
-          h = (g[q] - E(q) * h) * ri2;
-          ri = ri2;
-        } else {
-          ri = rcp(D(q));
-          h = g[q] * ri;
-        }
-      }
-    }
-    y0 = h;
-    v0 = a_first * ri;
   };
-  if (sizeof(T) == 4 && len == S) interface_data(S);
-  else if (len > 0) interface_data(len);
+  if (sizeof(T) == 4 && len == S) factor(S);
+  else if (len > 0) factor(len);
   // phase 2: reduced system X_p = (t_p, b_p) (first/last values of segment p)
   //   t_p + v0_p b_{p-1} + w0_p t_{p+1} = y0_p
   //   b_p + vl_p b_{p-1} + wl_p t_{p+1} = yl_p
-  // solved by parallel cyclic reduction over the segments: at distance d
-  // every segment substitutes b_{p-d} (row b of p-d) and t_{p+d} (row t of
-  // p+d); the couplings keep their shape (to b_{p-2d} and t_{p+2d} only), so
-  // after ceil(log2 P) steps every segment holds its own values.  Segments
-  // past either end of the column contribute zeros (the end couplings are
-  // zero: walls, and periodic wraps are lagged).  Every wave takes part.
+  // with (y, v, w) the segment's solutions for the three right-hand sides,
+  // taken at its first and last rows.  Parallel cyclic reduction over the
+  // segments: at distance d every segment substitutes b_{p-d} (row b of
+  // p-d) and t_{p+d} (row t of p+d); the couplings keep their shape (to
+  // b_{p-2d} and t_{p+2d} only), so after ceil(log2 P) steps every segment
+  // holds its own values.  Segments past either end of the column
+  // contribute zeros.  Every wave takes part.
   // FP32 double-buffers the state in LDS (one barrier per step); FP64 uses
   // one buffer and two barriers, keeping the block within 64 KiB of LDS.
   const int sl = p * 64 + lane;
   T bprev = T(0), tnext = T(0);
   constexpr int kZBuf = sizeof(T) == 8 ? 1 : 2;
   if (Pi > 1) {
+    T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
+    auto ends = [&](const int len) {
+      T yt = T(0), vt = T(0), wt = T(0);
+#pragma unroll
+      for (int q = S - 1; q >= 0; --q) {
+        if (q < len) {
+          if (q == len - 1) {
+            yt = g[q]; vt = vv[q]; wt = cp[q];
+            yl = yt; vl = vt; wl = wt;
+          } else {
+            yt = g[q] - cp[q] * yt;
+            vt = vv[q] - cp[q] * vt;
+            wt = -cp[q] * wt;
+          }
+        }
+      }
+      y0 = yt; v0 = vt; w0 = wt;
+    };
+    if (sizeof(T) == 4 && len == S) ends(S);
+    else if (len > 0) ends(len);
     const int PS = P * 64;
     T at = v0, ab = vl, ct = w0, cb = wl, rt = y0, rb = yl;
     int buf = 0;
@@ -448,41 +468,26 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     tnext = (p < Pi - 1) ? W[(p + 1) * 64 + lane] : T(0);
     __syncthreads();   // the group's LDS may be rewritten by its next call
   }
-  // phase 3: re-solve the segment with its end couplings on the right-hand side
-  auto resolve = [&](const int len) {
-    T cpa[S];
-    T cp = T(0), yf = T(0);
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      if (q < len) {
-        T rhs = g[q];
-        if (q == 0) rhs -= a_first * bprev;
-        if (q == len - 1) rhs -= c_last * tnext;
-        const T a = (q == 0) ? T(0) : A(q);
-        const T rd = rcp(D(q) - a * cp);
-        yf = (rhs - a * yf) * rd;
-        cp = (q == len - 1) ? T(0) : E(q) * rd;
-        g[q] = yf;
-        cpa[q] = cp;
-      }
-    }
+  // phase 3: back substitution of g - v bprev - w tnext
+  auto backsub = [&](const int len) {
     T x = T(0);
 #pragma unroll
     for (int q = S - 1; q >= 0; --q) {
       if (q < len) {
-        x = g[q] - cpa[q] * x;
-        if (live) u[(ks + q) * plane + own] = x;
+        if (q == len - 1) x = g[q] - vv[q] * bprev - cp[q] * tnext;
+        else x = (g[q] - vv[q] * bprev) - cp[q] * x;
+        if (live) st_row<T>(u + (ks + q) * plane, bown, x);
       }
     }
   };
-  if (sizeof(T) == 4 && len == S) resolve(S);
-  else if (len > 0) resolve(len);
+  if (sizeof(T) == 4 && len == S) backsub(S);
+  else if (len > 0) backsub(len);
 #undef A
 #undef E
 #undef D
 }
 
-template <class T, int S, int C>
+template <class T, int S, int C, bool NBR>
 __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                 int parity, int zero_guess) {
   extern __shared__ unsigned char lds_raw[];
@@ -491,8 +496,8 @@ __global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt,
   const DevLevel& L = Lt[ll.items[t]];
   // threadIdx.y is constant across a wave (blockDim.x == 64): say so to the compiler
   const int seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  zline_group<T, S, C>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
-                    threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
+  zline_group<T, S, C, NBR>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
+                            threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
 }
 
 template <class T> struct TailNz;
@@ -1500,12 +1505,17 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
   const dim3 blk(64, segs);
   // reduced-system state (two buffers in FP32, one in FP64) + staged rows
   const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
-  if (fp64)
-    hipLaunchKernelGGL((k_zline<double, kZSeg, kZChunk64>), dim3(ll.nblocks), blk, lds, s, L, ll, parity,
-                       zero_guess);
-  else
-    hipLaunchKernelGGL((k_zline<float, kZSeg, kZChunk32>), dim3(ll.nblocks), blk, lds, s, L, ll, parity,
-                       zero_guess);
+#define ZL(T, CH, NBR) \
+  hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess)
+  const bool nbr = !(zero_guess & 1);
+  if (fp64) {
+    if (nbr) ZL(double, kZChunk64, true);
+    else ZL(double, kZChunk64, false);
+  } else {
+    if (nbr) ZL(float, kZChunk32, true);
+    else ZL(float, kZChunk32, false);
+  }
+#undef ZL
 }
 void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                       int zero_guess, hipStream_t s) {

@@ -164,14 +164,18 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     d.vsum = I->H.vol_sum;
     d.vrsum = I->H.vol_rho_sum;
     d.nullsp = I->H.singular ? 1 : 0;
-    const size_t nco = H.co.size();
+    // the tables end with kZSeg zeros: the z-line sweep reads the rows of a
+    // whole segment from its first row on, past nz in the last one
+    const size_t nco = H.co.size() + kZSeg;
     double* c64 = (double*)I->mem.alloc(nco * 8);
     float* c32 = (float*)I->mem.alloc(nco * 4);
     double* vol = (double*)I->mem.alloc(H.vol.size() * 8);
     double* rho = (double*)I->mem.alloc(H.nz * 8);
     if (!c64 || !c32 || !vol || !rho) return fail();
-    std::vector<float> co32(H.co.begin(), H.co.end());
-    AMG_HIP(hipMemcpy(c64, H.co.data(), nco * 8, hipMemcpyHostToDevice));
+    std::vector<double> co64(H.co.begin(), H.co.end());
+    co64.resize(nco, 0.0);
+    std::vector<float> co32(co64.begin(), co64.end());
+    AMG_HIP(hipMemcpy(c64, co64.data(), nco * 8, hipMemcpyHostToDevice));
     AMG_HIP(hipMemcpy(c32, co32.data(), nco * 4, hipMemcpyHostToDevice));
     AMG_HIP(hipMemcpy(vol, H.vol.data(), H.vol.size() * 8, hipMemcpyHostToDevice));
     AMG_HIP(hipMemcpy(rho, H.rho.data() + 1, H.nz * 8, hipMemcpyHostToDevice));
