@@ -30,8 +30,11 @@ struct DevLevel {
   int ncell;                 // nx*ny*nz
   int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
   int nullsp;                // the operator is singular (periodic/Neumann): constants in the null space
+  int stop;                  // gated cycles: set once the stop rule held; the kernels then return
   unsigned mhx, mny;         // ceil(2^32/hx), ceil(2^32/ny) (0 for a divisor of 1): e/hx = umulhi(e, mhx)
                              // exactly while e*hx < 2^32 (the coarse tail's small levels)
+  unsigned gmhx, gmny;       // division of any 32-bit index by hx, ny: multiplier and
+  int gshx, gsny;            // shifts (s1 | s2 << 8) of gm_magic
   int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
   double vsum;               // finest: sum of cell volumes
   double vrsum;              // finest: sum of V*rho
@@ -57,6 +60,22 @@ struct LaunchList {
   int nblocks;
   int items[kMaxItems];        // item -> DevLevel index (or transfer index)
   int prefix[kMaxItems + 1];   // block prefix
+};
+
+// Gated cycles of one instance: the cycles of a batch are enqueued without a
+// host round trip between them.  k_gate_check ends every cycle: it records
+// the cycle's norms in hist and applies the stop rule of the host's norm
+// step to them; once it stops, it sets DevLevel::stop of the instance's
+// levels and every later kernel of the batch returns at once.  The host
+// then replays hist through its own norm step.
+constexpr int kGateHist = 64;
+struct CycleGate {
+  int stop;                    // set when the stop rule holds (or hist is full)
+  int count;                   // cycles run in this batch
+  int iters0;                  // cycles before the batch
+  int maxcy, rule, fp64, report, pad;
+  double bnorm, tol;
+  double hist[kGateHist][6];   // per cycle: the 4 norms of the cycle, and the 2 of relmax (FP64)
 };
 
 // Per-item caller arrays (reference-layout fields), by value.
@@ -185,6 +204,22 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 void launch_zero(double* a, int n, hipStream_t s);
 // weighted sums fold kWsumPer cells per thread: blocks per item = cdiv(ncell, kBlock*kWsumPer)
 constexpr int kWsumPer = 16;
+
+// Division by an invariant d >= 1 of any 32-bit n (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig.
+// 4.1): with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// t = mulhi(m, n) and q = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0).
+// Returns m; *sh packs the two shifts as s1 | s2 << 8.
+inline unsigned gm_magic(unsigned d, int* sh) {
+  int l = 0;
+  while ((1ull << l) < d) ++l;
+  const int s1 = l < 1 ? l : 1, s2 = l > 1 ? l - 1 : 0;
+  *sh = s1 | (s2 << 8);
+  return (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+}
+void launch_gate_check(CycleGate* g, const double* out, const double* outc, DevLevel* L, int nL,
+                       int inst, hipStream_t s);
+void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s);
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum,
                      double* out, int ostride, hipStream_t s);
 

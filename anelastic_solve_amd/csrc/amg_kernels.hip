@@ -11,6 +11,7 @@
 // serialise the loads of a line solve; the element routines are therefore
 // templates over the pointer types and the kernels cast the level
 // descriptors' pointers before use.  Chequer indices are 32-bit.
+#include "amg.h"
 #include "amg_internal.h"
 
 #include <cmath>
@@ -38,6 +39,9 @@ __device__ __forceinline__ int find_item(const int* prefix, int n, int b) {
   }
   return lo;
 }
+
+// A gated cycle has stopped (DevLevel::stop): the kernel returns at once.
+__device__ __forceinline__ bool level_off(const DevLevel& L) { return L.stop != 0; }
 
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);
 // remap so that every XCD walks a contiguous range of tiles and neighbouring
@@ -136,13 +140,21 @@ __device__ __forceinline__ Nb neighbours(const DevLevel& L, int i, int j, int k)
 }
 
 // Decode a chequer element index into (i, j, k).
+// floor(n / d) by the invariant divisor of a level (DevLevel::gmhx/gshx,
+// gmny/gsny: Granlund-Montgomery multiplier and shifts, exact for every
+// 32-bit n).
+__device__ __forceinline__ int gm_div(int n, unsigned m, int sh) {
+  const unsigned un = (unsigned)n, t = __umulhi(m, un);
+  return (int)((t + ((un - t) >> (sh & 0xff))) >> (sh >> 8));
+}
+
 __device__ __forceinline__ void decode(const DevLevel& L, int e, int& i, int& j, int& k) {
-  const int m = e % L.hx;
-  const int t = e / L.hx;
+  const int t = gm_div(e, L.gmhx, L.gshx);
+  const int m = e - t * L.hx;
   const int c = t & 1;
   const int row = t >> 1;
-  j = row % L.ny;
-  k = row / L.ny;
+  k = gm_div(row, L.gmny, L.gsny);
+  j = row - k * L.ny;
   i = 2 * m + ((c + j) & 1);
 }
 
@@ -194,8 +206,8 @@ __device__ __forceinline__ void rb_point_mjk(const DevLevel& L, UP u, FP f, CP c
 template <class T, class UP, class FP, class CP>
 __device__ __forceinline__ void rb_point_elem(const DevLevel& L, UP u, FP f, CP co, int e,
                                               int parity, int zero_guess) {
-  const int hx = L.hx, ny = L.ny;
-  const int m = e % hx, row = e / hx, j = row % ny, k = row / ny;
+  const int row = gm_div(e, L.gmhx, L.gshx), m = e - row * L.hx;
+  const int k = gm_div(row, L.gmny, L.gsny), j = row - k * L.ny;
   rb_point_mjk<T>(L, u, f, co, m, j, k, parity, zero_guess);
 }
 
@@ -218,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.hx * L.ny * L.nz)
     rb_point_elem<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), e, parity,
@@ -253,8 +266,9 @@ template <class T> __device__ __forceinline__ void st_row(sp<kG, T> row, unsigne
 }
 
 // NBR false: the other colour is zero (zero_guess bit 0, the first
-// half-sweep from u = 0) and is not read.
-template <class T, int S, int C, bool NBR>
+// half-sweep from u = 0) and is not read.  LEAN: the register-lean form of
+// the segment elimination (below).
+template <class T, int S, int C, bool NBR, bool LEAN>
 __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int zero_guess, int col0,
                                             int p, int lane, int P, T* __restrict__ lds, bool act) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
@@ -370,14 +384,24 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   const T a_first = A(0);                                  // coupling to u_{ks-1}
   // phase 1: forward elimination of the segment's rows for the data (g, in
   // place), the coupling a_first to u_{ks-1} entering row 0 (v) and the
-  // coupling E(len-1) to u_{ks+len} entering the last row; cp holds the
-  // eliminated super-diagonal, and cp[len-1] the last row's coupling (w).
+  // coupling E(len-1) to u_{ks+len} entering the last row (w).  The first
+  // values of the three solutions accumulate during the pass,
+  // x_0 = sum_q pi_q x'_q with pi_q = prod_{i<q} (-E(i) rd_i); the last
+  // values are the last row's.
+  // Short form: the eliminated v (vv) and super-diagonal (cp) are stored.
+  // LEAN form (two values per row instead of three, so a 1024-lane block
+  // fits in 64 registers and two blocks share a CU): only the pivot
+  // reciprocals rd are stored, and v and cp are recomputed from them with
+  // the same operations.  Both forms round alike, so every launch gives the
+  // same values whichever form it takes.
   // A full segment (n == S, every segment but the last of a ragged column)
   // instantiates the passes with a constant n, so the unrolled chains carry
   // no branches.
-  T vv[S], cp[S];
+  constexpr int SS = LEAN ? 1 : S, SL = LEAN ? S : 1;
+  T vv[SS], cp[SS], rdv[SL];
+  T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
   auto factor = [&](const int len) {
-    T cq = T(0), y = T(0), v = T(0);
+    T cq = T(0), y = T(0), v = T(0), pi = T(1);
 #pragma unroll
     for (int q = 0; q < S; ++q) {
       if (q < len) {
@@ -388,8 +412,16 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
         v = (q == 0) ? a_first * rd : -(a * rd) * v;
         cq = E(q) * rd;
         g[q] = y;
-        vv[q] = v;
-        cp[q] = cq;
+        if (q == 0) { y0 = y; v0 = v; }
+        else { y0 += pi * y; v0 += pi * v; }
+        if (q == len - 1) { yl = y; vl = v; wl = cq; w0 = pi * cq; }
+        else pi = (q == 0) ? -cq : pi * -cq;
+        if constexpr (LEAN) {
+          rdv[q] = rd;
+        } else {
+          vv[q] = v;
+          cp[q] = cq;
+        }
       }
     }
   };
@@ -411,26 +443,6 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   T bprev = T(0), tnext = T(0);
   constexpr int kZBuf = sizeof(T) == 8 ? 1 : 2;
   if (Pi > 1) {
-    T y0 = T(0), yl = T(0), v0 = T(0), vl = T(0), w0 = T(0), wl = T(0);
-    auto ends = [&](const int len) {
-      T yt = T(0), vt = T(0), wt = T(0);
-#pragma unroll
-      for (int q = S - 1; q >= 0; --q) {
-        if (q < len) {
-          if (q == len - 1) {
-            yt = g[q]; vt = vv[q]; wt = cp[q];
-            yl = yt; vl = vt; wl = wt;
-          } else {
-            yt = g[q] - cp[q] * yt;
-            vt = vv[q] - cp[q] * vt;
-            wt = -cp[q] * wt;
-          }
-        }
-      }
-      y0 = yt; v0 = vt; w0 = wt;
-    };
-    if (sizeof(T) == 4 && len == S) ends(S);
-    else if (len > 0) ends(len);
     const int PS = P * 64;
     T at = v0, ab = vl, ct = w0, cb = wl, rt = y0, rb = yl;
     int buf = 0;
@@ -468,14 +480,32 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     tnext = (p < Pi - 1) ? W[(p + 1) * 64 + lane] : T(0);
     __syncthreads();   // the group's LDS may be rewritten by its next call
   }
-  // phase 3: back substitution of g - v bprev - w tnext
+  // phase 3: back substitution of g - v bprev - w tnext (LEAN: v is
+  // recomputed from rd by the forward pass's own operations first)
   auto backsub = [&](const int len) {
+    if constexpr (LEAN) {
+      if (p > 0) {
+        T v = T(0);
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+          if (q < len) {
+            v = (q == 0) ? a_first * rdv[q] : -(A(q) * rdv[q]) * v;
+            g[q] -= v * bprev;
+          }
+        }
+      }
+    }
     T x = T(0);
 #pragma unroll
     for (int q = S - 1; q >= 0; --q) {
       if (q < len) {
-        if (q == len - 1) x = g[q] - vv[q] * bprev - cp[q] * tnext;
-        else x = (g[q] - vv[q] * bprev) - cp[q] * x;
+        if constexpr (LEAN) {
+          const T cq = E(q) * rdv[q];
+          x = (q == len - 1) ? g[q] - cq * tnext : g[q] - cq * x;
+        } else {
+          const T gq = g[q] - vv[q] * bprev;
+          x = (q == len - 1) ? gq - cp[q] * tnext : gq - cp[q] * x;
+        }
         if (live) st_row<T>(u + (ks + q) * plane, bown, x);
       }
     }
@@ -487,16 +517,19 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
 #undef D
 }
 
-template <class T, int S, int C, bool NBR>
-__global__ __launch_bounds__(1024) void k_zline(const DevLevel* __restrict__ Lt, LaunchList ll,
-                                                int parity, int zero_guess) {
+// The LEAN FP32 form is built for 8 waves per SIMD (64 registers), so two
+// 1024-lane blocks share a CU and one's loads overlap the other's solve.
+template <class T, int S, int C, bool NBR, bool LEAN>
+__global__ __launch_bounds__(1024, (LEAN && sizeof(T) == 4) ? 8 : 4) void k_zline(
+    const DevLevel* __restrict__ Lt, LaunchList ll, int parity, int zero_guess) {
   extern __shared__ unsigned char lds_raw[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   // threadIdx.y is constant across a wave (blockDim.x == 64): say so to the compiler
   const int seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  zline_group<T, S, C, NBR>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
+  zline_group<T, S, C, NBR, LEAN>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
                             threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
 }
 
@@ -613,6 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_zline_col(const DevLevel* __restrict
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.hx * L.ny)
     zline_column_any<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), parity,
@@ -628,7 +662,9 @@ __global__ __launch_bounds__(kBlock) void k_zline_col(const DevLevel* __restrict
 template <class T, int FX, int FY, int FZ, class UP, class FP, class CP, class VP, class CFP>
 __device__ __forceinline__ void restrict_elem(const DevLevel& F, UP u, FP f, CP co, VP vol,
                                               const DevLevel& C, CFP fc, int e) {
-  const int I = e % C.nx, JK = e / C.nx, J = JK % C.ny, K = JK / C.ny;
+  // nx = 2 hx: e / nx = (e >> 1) / hx
+  const int JK = gm_div(e >> 1, C.gmhx, C.gshx), I = e - JK * C.nx;
+  const int K = gm_div(JK, C.gmny, C.gsny), J = JK - K * C.ny;
   const auto vx = vol;
   const auto vy = vx + F.nx;
   const auto vz = vy + F.ny;
@@ -678,6 +714,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
   const DevXfer x = X[ll.items[t]];
   const DevLevel& C = Lt[x.coarse];
   const DevLevel& F = Lt[x.fine];
+  if (level_off(F)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < C.ncell)
     restrict_any<T>(F, gfield<const T>((const void*)F.u), gfield<const T>((const void*)F.f),
@@ -834,6 +871,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
   const DevXfer x = X[ll.items[t]];
   const DevLevel& F = Lt[x.fine];
   const DevLevel& C = Lt[x.coarse];
+  if (level_off(F)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < prolong_duals(F, C, x))
     prolong_any<T>(F, gfield<T>(F.u), C, gfield<const T>((const void*)C.u), x, e);
@@ -863,6 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse(const DevLevel* __restrict__ 
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const int r = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (r < L.ncell) coarse_row<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), gp(L.minvT), r);
 }
@@ -902,6 +941,7 @@ __global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) v
   extern __shared__ unsigned char lds_raw[];
   __shared__ DevLevel Ls[kTailMaxLevels];
   const DevTail& tl = Tt[ll.items[blockIdx.x]];
+  if (level_off(Lt[tl.lev[0]])) return;
   const int tid = threadIdx.y * 64 + threadIdx.x, nthr = 64 * blockDim.y;
   const int nl = tl.nlev;
   const sp<kS, unsigned char> base = (sp<kS, unsigned char>)lds_raw;
@@ -1002,6 +1042,7 @@ __global__ __launch_bounds__(kBlock) void k_change_max(const DevLevel* __restric
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const auto u = gfield<const double>((const void*)L.u);
   const auto u2 = gfield<const double>((const void*)L.u2);
   fine_loop<2>(L, b - ll.prefix[t], partial, b, [&](int e, double* v) {
@@ -1051,16 +1092,27 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     double xm, xp, ym, yp, fb, pu;
     float xme, xpe, yme, ype, pe;
   };
+  // rows are addressed as a plane base plus per-lane byte offsets that are
+  // the same in every plane (no vector address arithmetic per load)
+  const unsigned dxm = (unsigned)oxm * 8u, dxp = (unsigned)oxp * 8u, dym = (unsigned)oym * 8u,
+                 dyp = (unsigned)oyp * 8u, dc = (unsigned)col * 8u;
+  const unsigned sxm = (unsigned)oxm * 4u, sxp = (unsigned)oxp * 4u, sym = (unsigned)oym * 4u,
+                 syp = (unsigned)oyp * 4u, sc = (unsigned)col * 4u;
   auto load = [&](int k) {
     Plane q;
     const int pb = k * plane;
     const int kp = (k == nz - 1) ? 0 : k + 1;
-    q.xm = u[pb + oxm]; q.xp = u[pb + oxp]; q.ym = u[pb + oym]; q.yp = u[pb + oyp];
-    q.fb = f[pb + col] - cs;      // cs: the compatibility shift of b (first defect only)
-    q.pu = u[kp * plane + col];
+    const sp<kG, const double> ur = u + pb;
+    q.xm = ld_row<double>(ur, dxm); q.xp = ld_row<double>(ur, dxp);
+    q.ym = ld_row<double>(ur, dym); q.yp = ld_row<double>(ur, dyp);
+    // cs: the compatibility shift of b (first defect only)
+    q.fb = ld_row<double>((sp<kG, const double>)(f + pb), dc) - cs;
+    q.pu = ld_row<double>(u + kp * plane, dc);
     if constexpr (MIXED) {
-      q.xme = ev[pb + oxm]; q.xpe = ev[pb + oxp]; q.yme = ev[pb + oym]; q.ype = ev[pb + oyp];
-      q.pe = ev[kp * plane + col];
+      const sp<kG, const float> er = ev + pb;
+      q.xme = ld_row<float>(er, sxm); q.xpe = ld_row<float>(er, sxp);
+      q.yme = ld_row<float>(er, sym); q.ype = ld_row<float>(er, syp);
+      q.pe = ld_row<float>(ev + kp * plane, sc);
     } else {
       q.xme = q.xpe = q.yme = q.ype = q.pe = 0.0f;
     }
@@ -1076,7 +1128,7 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
   double vc = sum(u[k0 * plane + col], ec0);
   float ec = ec0;
   auto step = [&](int k, const Plane& cur) {
-    const int e = k * plane + col;
+    const int pbk = k * plane;
     const double vp = sum(cur.pu, cur.pe);
     double s = wxm * sum(cur.xm, cur.xme) + wxp * sum(cur.xp, cur.xpe);
     s += wym * sum(cur.ym, cur.yme);
@@ -1085,15 +1137,15 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     s += cz[nz + k] * vp;
     const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
     const double res = cur.fb - (s + diag * vc);
-    if constexpr (SHIFT) f[e] = cur.fb;
+    if constexpr (SHIFT) st_row<double>(f + pbk, dc, cur.fb);
     if constexpr (MODE == kCorrDefect32) {
-      pout[e] = vc;
-      r32[e] = (float)res;
+      st_row<double>(pout + pbk, dc, vc);
+      st_row<float>(r32 + pbk, sc, (float)res);
       v[0] = amax(v[0], res);
       v[1] = amax(v[1], (double)ec);
       v[2] = amax(v[2], vc);
     } else {
-      if constexpr (MODE == kDefect32) r32[e] = (float)res;
+      if constexpr (MODE == kDefect32) st_row<float>(r32 + pbk, sc, (float)res);
       v[0] = amax(v[0], res);
       v[1] = amax(v[1], cur.fb);
     }
@@ -1142,6 +1194,7 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const DevLevel& S = Lt[L.link];
   const int hx = L.hx, nx = L.nx, ny = L.ny, nz = L.nz;
   const int plane = 2 * hx * ny;
@@ -1182,14 +1235,25 @@ __global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double
   const int t = blockIdx.x;
   const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  // one pass over the partials for every value (nvals <= 4)
+  // one pass over the partials for every value (nvals <= 4), kFinU
+  // partials per thread and round: their loads (clamped into the item, so
+  // unconditional) are issued before they are folded, in order
+  constexpr int kFinU = 8;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = b0 + threadIdx.x; b < b1; b += kBlock) {
+  for (int b = b0 + threadIdx.x; b < b1; b += kBlock * kFinU) {
+    double x[kFinU][4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (q < nvals) {
-        const double x = partial[(size_t)b * nvals + q];
-        acc[q] = op_sum ? acc[q] + x : nmax(acc[q], x);
+    for (int r = 0; r < kFinU; ++r) {
+      const int bb = min(b + r * kBlock, b1 - 1);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[r][q] = partial[(size_t)bb * nvals + min(q, nvals - 1)];
+    }
+#pragma unroll
+    for (int r = 0; r < kFinU; ++r) {
+      if (b + r * kBlock < b1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < nvals) acc[q] = op_sum ? acc[q] + x[r][q] : nmax(acc[q], x[r][q]);
       }
     }
   }
@@ -1248,16 +1312,35 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const DevLevel* __restrict__ 
   const auto f = gfield<double>(L.f);
   const auto u = gfield<double>(L.u);
   double v[1] = {0.0};
-  for (int s = 0; s < kWsumPer; ++s) {
-    const int e = e0 + s * kBlock;
-    if (e < L.ncell) {
+  // every load is unconditional (indices clamped into the item), so each
+  // chunk of kIngestU cells issues its loads before their uses; the cells
+  // past the item's end are masked at the stores and the sum
+  constexpr int kIngestU = 4;
+#pragma unroll
+  for (int s0 = 0; s0 < kWsumPer; s0 += kIngestU) {
+    if (s0 > 0) __builtin_amdgcn_sched_barrier(0);
+    double sv[kIngestU], pv[kIngestU], w[kIngestU], fb[kIngestU];
+#pragma unroll
+    for (int q = 0; q < kIngestU; ++q) {
+      const int e = e0 + (s0 + q) * kBlock;
       int i, j, k;
-      decode(L, e, i, j, k);
+      decode(L, e < L.ncell ? e : L.ncell - 1, i, j, k);
       const size_t fi = fidx(i, j, k, L.nx, L.ny);
-      const double bv = src[fi] + ((bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k]);
-      f[e] = bv;
-      u[e] = ph[fi];
-      v[0] += (vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i]) * bv;
+      sv[q] = src[fi];
+      pv[q] = ph[fi];
+      fb[q] = (bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k];
+      w[q] = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
+    }
+#pragma unroll
+    for (int q = 0; q < kIngestU; ++q) {
+      const int e = e0 + (s0 + q) * kBlock;
+      const bool in = e < L.ncell;
+      const double bv = sv[q] + fb[q];
+      if (in) {
+        f[e] = bv;
+        u[e] = pv[q];
+      }
+      v[0] += in ? w[q] * bv : 0.0;
     }
   }
   block_reduce_store<1, true>(v, partial + b);
@@ -1275,14 +1358,24 @@ __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt
   const auto rho = gp(L.rho);
   const auto x = gfield<const double>((const void*)(which == 0 ? L.u : L.f));
   double v[1] = {0.0};
-  for (int s = 0; s < kWsumPer; ++s) {
-    const int e = e0 + s * kBlock;
-    if (e < L.ncell) {
+  // unconditional loads (clamped indices), a chunk of kWsumU cells issued
+  // before their uses, as in k_ingest; cells past the item's end add zero
+  constexpr int kWsumU = 8;
+#pragma unroll
+  for (int s0 = 0; s0 < kWsumPer; s0 += kWsumU) {
+    if (s0 > 0) __builtin_amdgcn_sched_barrier(0);
+    double w[kWsumU], xv[kWsumU];
+#pragma unroll
+    for (int q = 0; q < kWsumU; ++q) {
+      const int e = e0 + (s0 + q) * kBlock;
+      const int ec = e < L.ncell ? e : L.ncell - 1;
       int i, j, k;
-      decode(L, e, i, j, k);
-      const double w = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
-      v[0] += w * ((which == 2) ? rho[k] : x[e]);
+      decode(L, ec, i, j, k);
+      w[q] = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
+      xv[q] = (which == 2) ? rho[k] : x[ec];
     }
+#pragma unroll
+    for (int q = 0; q < kWsumU; ++q) v[0] += (e0 + (s0 + q) * kBlock < L.ncell) ? w[q] * xv[q] : 0.0;
   }
   block_reduce_store<1, true>(v, partial + b);
 }
@@ -1314,6 +1407,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_u(const DevLevel* __restrict__ 
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   if (active && !active[L.inst]) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e >= L.ncell) return;
@@ -1325,6 +1419,7 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(const DevLevel* __restrict
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.ncell) gfield<T>(L.u)[e] = T(0);
 }
@@ -1492,8 +1587,64 @@ inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) /
 
 }  // namespace
 
+// End of a gated cycle (one thread): record the cycle's norms and apply the
+// stop rule of the host's norm step (Context::post_norms) to them, with the
+// same double operations, so the host's replay reaches the same decision.
+// On a stop it sets the stop word of every level descriptor of instance
+// inst: the kernels of the later cycles of the batch read it with the rest
+// of their descriptor and return at once.
+__global__ void k_gate_check(CycleGate* __restrict__ g, const double* __restrict__ out,
+                             const double* __restrict__ outc, DevLevel* __restrict__ Lt, int nL,
+                             int inst) {
+  if (threadIdx.x != 0 || g->stop) return;
+  const int n = g->count;
+  double* h = g->hist[n];
+  for (int q = 0; q < 4; ++q) h[q] = out[q];
+  h[4] = g->report ? outc[0] : 0.0;
+  h[5] = g->report ? outc[1] : 0.0;
+  const double rmax = out[0];
+  double relmax;
+  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
+  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
+  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
+  g->count = n + 1;
+  const int iters = g->iters0 + n + 1;
+  int stop = 0;
+  if (!isfinite(rmax)) {
+    stop = 1;
+  } else {
+    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
+    if (crit <= g->tol || iters >= g->maxcy) stop = 1;
+  }
+  if (n + 1 >= kGateHist) stop = 1;   // history full: the host runs the next batch
+  g->stop = stop;
+  if (stop)
+    for (int l = 0; l < nL; ++l)
+      if (Lt[l].inst == inst) Lt[l].stop = 1;
+}
+
+// Clears the stop words of instance inst after a batch.
+__global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, int inst) {
+  for (int l = threadIdx.x; l < nL; l += blockDim.x)
+    if (Lt[l].inst == inst) Lt[l].stop = 0;
+}
+
 // ---- launchers --------------------------------------------------------------
 #define LAUNCH(kern, grid, blk, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(blk), 0, s, __VA_ARGS__)
+
+// Compute units of the current device (cached per device).
+static int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
 
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                      hipStream_t s) {
@@ -1505,15 +1656,23 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
   const dim3 blk(64, segs);
   // reduced-system state (two buffers in FP32, one in FP64) + staged rows
   const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
-#define ZL(T, CH, NBR) \
-  hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, zero_guess)
+#define ZL(T, CH, NBR, LEAN)                                                                      \
+  hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR, LEAN>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, \
+                     zero_guess)
   const bool nbr = !(zero_guess & 1);
+  // FP64 always takes the lean form (the short one does not fit its
+  // registers).  FP32 takes it when there are more blocks than CUs, so that
+  // blocks can pair up; a smaller launch is latency-bound and takes the
+  // shorter dependent chains of the short form.
   if (fp64) {
-    if (nbr) ZL(double, kZChunk64, true);
-    else ZL(double, kZChunk64, false);
+    if (nbr) ZL(double, kZChunk64, true, true);
+    else ZL(double, kZChunk64, false, true);
+  } else if (ll.nblocks > device_cus()) {
+    if (nbr) ZL(float, kZChunk32, true, true);
+    else ZL(float, kZChunk32, false, true);
   } else {
-    if (nbr) ZL(float, kZChunk32, true);
-    else ZL(float, kZChunk32, false);
+    if (nbr) ZL(float, kZChunk32, true, false);
+    else ZL(float, kZChunk32, false, false);
   }
 #undef ZL
 }
@@ -1624,6 +1783,13 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 }
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
+}
+void launch_gate_check(CycleGate* g, const double* out, const double* outc, DevLevel* L, int nL,
+                       int inst, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate_check, dim3(1), dim3(64), 0, s, g, out, outc, L, nL, inst);
+}
+void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(64), 0, s, L, nL, inst);
 }
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
                      int ostride, hipStream_t s) {

@@ -164,7 +164,7 @@ void amg_server_s::run() {
     for (const Slot& x : A)
       if (!x.inst->done) C.push_back(x.inst);
     if (rc == AMG_OK && !C.empty()) {
-      rc = ctx.cycle_and_norms(C);
+      rc = ctx.cycle_and_norms(C, false);
       if (rc == AMG_OK && hipStreamSynchronize(ctx.stream) != hipSuccess) rc = AMG_EHIP;
       if (rc == AMG_OK) ctx.post_norms(C);
     }

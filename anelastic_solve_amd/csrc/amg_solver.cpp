@@ -8,6 +8,7 @@
 #include "amg_solver.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -88,6 +89,8 @@ int Context::init(const amg_config& c) {
   AMG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   AMG_HIP(hipHostMalloc((void**)&h_out, sizeof(double) * kOutRegion * 4, hipHostMallocDefault));
   AMG_HIP(hipMalloc((void**)&d_out, sizeof(double) * kOutRegion * 4));
+  AMG_HIP(hipHostMalloc((void**)&h_gate, sizeof(CycleGate), hipHostMallocDefault));
+  AMG_HIP(hipMalloc((void**)&d_gate, sizeof(CycleGate)));
   for (auto& e : ev) AMG_HIP(hipEventCreate(&e));
   return AMG_OK;
 }
@@ -104,6 +107,8 @@ Context::~Context() {
   if (d_partial) (void)hipFree(d_partial);
   if (d_out) (void)hipFree(d_out);
   if (h_out) (void)hipHostFree(h_out);
+  if (d_gate) (void)hipFree(d_gate);
+  if (h_gate) (void)hipHostFree(h_gate);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -152,6 +157,8 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     };
     d.mhx = magic((unsigned)d.hx);
     d.mny = magic((unsigned)d.ny);
+    d.gmhx = gm_magic((unsigned)d.hx, &d.gshx);
+    d.gmny = gm_magic((unsigned)d.ny, &d.gsny);
     d.px = I->H.bc[0][0] == AMG_BC_PERIODIC;
     d.py = I->H.bc[1][0] == AMG_BC_PERIODIC;
     d.pz = I->H.bc[2][0] == AMG_BC_PERIODIC;
@@ -511,8 +518,10 @@ std::vector<Item> march_items(const Context& C, const std::vector<Instance*>& S,
 }  // namespace
 
 // Launches one cycle on every instance of S plus the norm kernels and the
-// device->host copy of the per-instance results (h_out, nvals per item).
-int Context::cycle_and_norms(const std::vector<Instance*>& S) {
+// device->host copy of the per-instance results (h_out, nvals per item);
+// gated (one instance): the gate check records the norms on the device
+// instead of the copy.
+int Context::cycle_and_norms(const std::vector<Instance*>& S, bool gated) {
   if (S.empty()) return AMG_OK;
   const bool fp64 = S[0]->fp64;
   const bool relmax = fp64 && cfg.report_relmax;
@@ -540,6 +549,10 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S) {
       launch_change_max(dL, ll, part, stream);
       launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
     });
+  }
+  if (gated) {
+    launch_gate_check(d_gate, d_out, d_out + kOutRegion * 2, dL, (int)hL.size(), S[0]->id, stream);
+    return AMG_OK;
   }
   (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
   if (relmax)
@@ -696,6 +709,85 @@ int Context::finish(const std::vector<Instance*>& S) {
   return AMG_OK;
 }
 
+// The cycles of one instance in gated batches: a batch enqueues the cycle
+// graphs of as many cycles as the instance's previous solve took (one when
+// there is none) without a host round trip between them; each cycle ends
+// with k_gate_check, and once it stops the batch, the rest of the batch's
+// kernels return at once.  After the batch the host replays the recorded
+// norms through post_norms, which must reach the same decision at the same
+// cycle.  The graphs (FP64; mixed for each parity of the current FP64
+// buffer) are captured with the gate in their launch lists.  When capture
+// is unavailable the caller's ungated loop takes over.
+int Context::gated_cycles(Instance* I, double* bytes) {
+  const std::vector<Instance*> A{I};
+  while (!I->done && I->iters < cfg.maxcy) {
+    int batch = I->last_iters > I->iters ? I->last_iters - I->iters : 1;
+    batch = std::min(batch, std::min(kGateHist, cfg.maxcy - I->iters));
+    // the graphs this batch needs
+    for (int b = 0; b < std::min(batch, 2); ++b) {
+      const int par = (I->cur + b) & 1;
+      const int gi = I->fp64 ? 0 : 1 + par;
+      if (gexec[gi]) continue;
+      const int cur0 = I->cur;
+      I->cur = par;
+      hipGraph_t g = nullptr;
+      bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+      if (ok) {
+        cycle_and_norms(A, true);
+        ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
+      }
+      if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
+      if (g) (void)hipGraphDestroy(g);
+      I->cur = cur0;
+      if (!ok) {
+        graph_ok = false;
+        gexec[gi] = nullptr;
+        (void)hipGetLastError();
+        return AMG_OK;
+      }
+    }
+    CycleGate& h = *h_gate;
+    h.stop = 0;
+    h.count = 0;
+    h.iters0 = I->iters;
+    h.maxcy = cfg.maxcy;
+    h.rule = cfg.stop_rule;
+    h.fp64 = I->fp64 ? 1 : 0;
+    h.report = (I->fp64 && cfg.report_relmax) ? 1 : 0;
+    h.pad = 0;
+    h.bnorm = I->bnorm;
+    h.tol = cfg.tol;
+    AMG_HIP(hipMemcpyAsync(d_gate, h_gate, offsetof(CycleGate, hist), hipMemcpyHostToDevice, stream));
+    for (int b = 0; b < batch; ++b)
+      AMG_HIP(hipGraphLaunch(gexec[I->fp64 ? 0 : 1 + ((I->cur + b) & 1)], stream));
+    AMG_HIP(hipMemcpyAsync(h_gate, d_gate, sizeof(CycleGate), hipMemcpyDeviceToHost, stream));
+    launch_gate_reset(dL, (int)hL.size(), I->id, stream);
+    AMG_HIP(hipStreamSynchronize(stream));
+    if (h.count < 1 || h.count > batch) {
+      set_error("gated cycles: the device ran " + std::to_string(h.count) + " of " +
+                std::to_string(batch) + " cycles");
+      return AMG_EHIP;
+    }
+    for (int c = 0; c < h.count; ++c) {
+      if (I->done) {
+        set_error("gated cycles: the device ran past the host's stop");
+        return AMG_EHIP;
+      }
+      for (int q = 0; q < 4; ++q) h_out[q] = h.hist[c][q];
+      h_out[kOutRegion * 2] = h.hist[c][4];
+      h_out[kOutRegion * 2 + 1] = h.hist[c][5];
+      post_norms(A);
+      *bytes += (double)I->bytes_per_cycle;
+    }
+    const bool full = h.count == kGateHist;
+    if ((h.stop != 0) != I->done && !(full && !I->done)) {
+      set_error("gated cycles: the device and the host stop at different cycles");
+      return AMG_EHIP;
+    }
+  }
+  return AMG_OK;
+}
+
 int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
   if (S.empty()) return AMG_OK;
   AMG_HIP(hipEventRecord(ev[2], stream));
@@ -704,6 +796,10 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
   AMG_HIP(hipEventRecord(ev[0], stream));
   const bool single = S.size() == 1 && cfg.use_graph;
   double bytes = 0.0;
+  if (single && graph_ok) {
+    rc = gated_cycles(S[0], &bytes);
+    if (rc != AMG_OK) return rc;
+  }
   for (int it = 0; it < cfg.maxcy; ++it) {
     std::vector<Instance*> A;
     for (Instance* I : S)
@@ -716,7 +812,7 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
         hipGraph_t g = nullptr;
         bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
-          cycle_and_norms(A);
+          cycle_and_norms(A, false);
           ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
         }
         if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
@@ -728,15 +824,16 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
         }
       }
       if (gexec[gi]) AMG_HIP(hipGraphLaunch(gexec[gi], stream));
-      else cycle_and_norms(A);
+      else cycle_and_norms(A, false);
     } else {
-      cycle_and_norms(A);
+      cycle_and_norms(A, false);
     }
     AMG_HIP(hipStreamSynchronize(stream));
     post_norms(A);
     for (Instance* I : A) bytes += (double)I->bytes_per_cycle;
   }
   AMG_HIP(hipEventRecord(ev[1], stream));
+  for (Instance* I : S) I->last_iters = I->iters;
   rc = finish(S);
   if (rc != AMG_OK) return rc;
   AMG_HIP(hipEventRecord(ev[3], stream));

@@ -64,6 +64,7 @@ struct Instance {
   // per-solve state
   bool converged = false, done = false;
   int iters = 0;
+  int last_iters = 0;            // cycles of the previous solve (the gated batch size)
   double bnorm = 0.0, relres = 0.0, relmax = 0.0, res0 = 0.0;
   int status = AMG_OK;
   size_t bytes_per_cycle = 0;
@@ -88,6 +89,8 @@ struct Context {
   size_t cap_partial = 0;
   double* d_out = nullptr;       // per-item results
   double* h_out = nullptr;       // pinned mirror
+  CycleGate* d_gate = nullptr;   // gated cycles: device control block
+  CycleGate* h_gate = nullptr;   // and its pinned mirror
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // single-instance graphs: [0] fp64 cycle, [1]/[2] mixed cycle for cur = 0/1
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
@@ -104,7 +107,8 @@ struct Context {
   int prepare(const std::vector<Instance*>& S);
   const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
-  int cycle_and_norms(const std::vector<Instance*>& S);
+  int cycle_and_norms(const std::vector<Instance*>& S, bool gated);
+  int gated_cycles(Instance* I, double* bytes);
   void post_norms(const std::vector<Instance*>& S);
   int run_kernel(Instance* I, int which, int level, int reps, double* bytes);
   int level_field(Instance* I, int level, int field, int dir, double* buf);

@@ -227,3 +227,43 @@ def test_server_equals_alone(cuda):
             s.mg_solver(p, rhs)
             assert torch.equal(p, outs[k]), (q, k)
     srv.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gated_batches_equal_fresh_solves(cuda, precision):
+    """A solver enqueues the cycles of a solve in batches sized by its
+    previous solve's cycle count, each cycle ending in a device-side stop
+    check after which the rest of the batch returns at once.  Reused on a
+    hard and then an easy solve (the batch overshoots) and the reverse (it
+    undershoots), it gives bitwise what a fresh solver (one-cycle batches)
+    gives: the same pressure, cycle count and residual."""
+    torch = cuda
+    from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
+    g = box_grid(48, 32, 40, beta=1.5)
+    b1 = torch.from_numpy(workloads.psource_host(*workloads.velocity_white(g, 31), g)).cuda()
+    rng = np.random.Generator(np.random.PCG64(32))
+    b2 = b1 + torch.from_numpy(1e-7 * rng.standard_normal(g.shape)).cuda()
+    cfg = lambda: default_config(precision=precision, tol=1e-10)
+    warm = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
+    PressureSolver(g, cfg()).mg_solver(warm, b1)       # the converged field: a warm start for b2
+    cases = {"hard": (b1, torch.zeros_like(warm)), "easy": (b2, warm)}
+
+    def run(s, name):
+        b, p0 = cases[name]
+        p = p0.clone()
+        it, _ = s.mg_solver(p, b)
+        return p, it, s.stats.relres
+
+    fresh = {}
+    for name in cases:
+        s = PressureSolver(g, cfg())
+        fresh[name] = run(s, name)
+        s.close()
+    assert fresh["hard"][1] > fresh["easy"][1] >= 1, (fresh["hard"][1], fresh["easy"][1])
+    for order in (("hard", "easy", "hard"), ("easy", "hard", "easy")):
+        s = PressureSolver(g, cfg())
+        for name in order:
+            p, it, rr = run(s, name)
+            assert it == fresh[name][1] and rr == fresh[name][2], (order, name, it, fresh[name][1])
+            assert torch.equal(p, fresh[name][0]), (order, name)
+        s.close()
