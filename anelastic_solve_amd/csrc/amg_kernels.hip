@@ -264,6 +264,14 @@ template <class T> __device__ __forceinline__ T ld_row(sp<kG, const T> row, unsi
 template <class T> __device__ __forceinline__ void st_row(sp<kG, T> row, unsigned boff, T v) {
   *(sp<kG, T>)((sp<kG, char>)row + boff) = v;
 }
+// The same, non-temporal: for data that is not read again before it would
+// have left the L2 (the streaming stages of the mixed outer step).
+template <class T> __device__ __forceinline__ void st_row_nt(sp<kG, T> row, unsigned boff, T v) {
+  __builtin_nontemporal_store(v, (T*)(sp<kG, T>)((sp<kG, char>)row + boff));
+}
+template <class T> __device__ __forceinline__ T ld_row_nt(sp<kG, const T> row, unsigned boff) {
+  return __builtin_nontemporal_load((const T*)(sp<kG, const T>)((sp<kG, const char>)row + boff));
+}
 
 // NBR false: the other colour is zero (zero_guess bit 0, the first
 // half-sweep from u = 0) and is not read.  LEAN: the register-lean form of
@@ -1105,8 +1113,11 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     const sp<kG, const double> ur = u + pb;
     q.xm = ld_row<double>(ur, dxm); q.xp = ld_row<double>(ur, dxp);
     q.ym = ld_row<double>(ur, dym); q.yp = ld_row<double>(ur, dyp);
-    // cs: the compatibility shift of b (first defect only)
-    q.fb = ld_row<double>((sp<kG, const double>)(f + pb), dc) - cs;
+    // cs: the compatibility shift of b (first defect only).  The mixed step
+    // reads b once per cell and nothing else reads it during the step: a
+    // non-temporal load (it was measured slower in the FP64 residual)
+    const sp<kG, const double> fr = (sp<kG, const double>)(f + pb);
+    q.fb = (MIXED ? ld_row_nt<double>(fr, dc) : ld_row<double>(fr, dc)) - cs;
     q.pu = ld_row<double>(u + kp * plane, dc);
     if constexpr (MIXED) {
       const sp<kG, const float> er = ev + pb;
@@ -1139,8 +1150,10 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     const double res = cur.fb - (s + diag * vc);
     if constexpr (SHIFT) st_row<double>(f + pbk, dc, cur.fb);
     if constexpr (MODE == kCorrDefect32) {
-      st_row<double>(pout + pbk, dc, vc);
-      st_row<float>(r32 + pbk, sc, (float)res);
+      // the new p is read in the next cycle and the FP32 defect by the next
+      // V-cycle's first sweep, both after many more bytes than the L2 holds
+      st_row_nt<double>(pout + pbk, dc, vc);
+      st_row_nt<float>(r32 + pbk, sc, (float)res);
       v[0] = amax(v[0], res);
       v[1] = amax(v[1], (double)ec);
       v[2] = amax(v[2], vc);
@@ -1296,51 +1309,71 @@ __global__ __launch_bounds__(kBlock) void k_from_fortran(const DevLevel* __restr
 }
 
 // Ingest of a solve: b = rhs + boundary fold and u = phi (reference layout
-// -> chequer), with the partial sum of V*b of the block's cells (each thread
-// folds kWsumPer cells in a fixed order).
+// -> chequer), with the partial sum of V*b of the block's cells.  A thread
+// takes cell pairs (2m, j, k), (2m+1, j, k): one decode per pair, the pair's
+// two reference-layout values are adjacent (a wave reads contiguous rows),
+// and the two cells go to the two colours' half-rows at position m.  Each
+// thread folds kWsumPer cells (kWsumPer/2 pairs) in a fixed order.
 __global__ __launch_bounds__(kBlock) void k_ingest(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                    PtrList rhs, PtrList phi,
                                                    double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
+  constexpr int kPairs = kWsumPer / 2;
+  const int np = L.ncell >> 1;                        // pairs: hx * ny * nz
+  const int p0 = (b - ll.prefix[t]) * kBlock * kPairs + threadIdx.x;
   const auto vol = gp(L.vol);
   const auto bf = gp(L.bcfold);
   const auto src = gp(rhs.p[t]);
   const auto ph = gp(phi.p[t]);
   const auto f = gfield<double>(L.f);
   const auto u = gfield<double>(L.u);
+  const int nx = L.nx, ny = L.ny, hx = L.hx;
   double v[1] = {0.0};
-  // every load is unconditional (indices clamped into the item), so each
-  // chunk of kIngestU cells issues its loads before their uses; the cells
+  // every load is unconditional (pair indices clamped into the item), so
+  // each chunk of kIngestU pairs issues its loads before their uses; pairs
   // past the item's end are masked at the stores and the sum
   constexpr int kIngestU = 4;
 #pragma unroll
-  for (int s0 = 0; s0 < kWsumPer; s0 += kIngestU) {
+  for (int s0 = 0; s0 < kPairs; s0 += kIngestU) {
     if (s0 > 0) __builtin_amdgcn_sched_barrier(0);
-    double sv[kIngestU], pv[kIngestU], w[kIngestU], fb[kIngestU];
+    double s0v[kIngestU], s1v[kIngestU], q0v[kIngestU], q1v[kIngestU], fb0[kIngestU], fb1[kIngestU];
+    double w0[kIngestU], w1[kIngestU];
+    int e0[kIngestU], e1[kIngestU];
 #pragma unroll
     for (int q = 0; q < kIngestU; ++q) {
-      const int e = e0 + (s0 + q) * kBlock;
-      int i, j, k;
-      decode(L, e < L.ncell ? e : L.ncell - 1, i, j, k);
-      const size_t fi = fidx(i, j, k, L.nx, L.ny);
-      sv[q] = src[fi];
-      pv[q] = ph[fi];
-      fb[q] = (bf[i] + bf[L.nx + j]) + bf[L.nx + L.ny + k];
-      w[q] = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
+      const int pp = p0 + (s0 + q) * kBlock;
+      const int pc = pp < np ? pp : np - 1;
+      const int row = gm_div(pc, L.gmhx, L.gshx), m = pc - row * hx;
+      const int k = gm_div(row, L.gmny, L.gsny), j = row - k * ny;
+      const int i = 2 * m;
+      const size_t fi = fidx(i, j, k, nx, ny);
+      s0v[q] = src[fi];
+      s1v[q] = src[fi + 1];
+      q0v[q] = ph[fi];
+      q1v[q] = ph[fi + 1];
+      fb0[q] = (bf[i] + bf[nx + j]) + bf[nx + ny + k];
+      fb1[q] = (bf[i + 1] + bf[nx + j]) + bf[nx + ny + k];
+      const double wyz = vol[nx + ny + k] * vol[nx + j];
+      w0[q] = wyz * vol[i];
+      w1[q] = wyz * vol[i + 1];
+      const int rb = (k * ny + j) * 2;
+      e0[q] = (rb + (j & 1)) * hx + m;                 // colour of (2m, j): j & 1
+      e1[q] = (rb + ((j + 1) & 1)) * hx + m;
     }
 #pragma unroll
     for (int q = 0; q < kIngestU; ++q) {
-      const int e = e0 + (s0 + q) * kBlock;
-      const bool in = e < L.ncell;
-      const double bv = sv[q] + fb[q];
+      const bool in = p0 + (s0 + q) * kBlock < np;
+      const double bv0 = s0v[q] + fb0[q], bv1 = s1v[q] + fb1[q];
       if (in) {
-        f[e] = bv;
-        u[e] = pv[q];
+        f[e0[q]] = bv0;
+        f[e1[q]] = bv1;
+        u[e0[q]] = q0v[q];
+        u[e1[q]] = q1v[q];
       }
-      v[0] += in ? w[q] * bv : 0.0;
+      v[0] += in ? w0[q] * bv0 : 0.0;
+      v[0] += in ? w1[q] * bv1 : 0.0;
     }
   }
   block_reduce_store<1, true>(v, partial + b);
