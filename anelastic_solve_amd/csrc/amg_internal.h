@@ -63,11 +63,11 @@ struct LaunchList {
 };
 
 // Gated cycles of one instance: the cycles of a batch are enqueued without a
-// host round trip between them.  k_gate_check ends every cycle: it records
-// the cycle's norms in hist and applies the stop rule of the host's norm
-// step to them; once it stops, it sets DevLevel::stop of the instance's
-// levels and every later kernel of the batch returns at once.  The host
-// then replays hist through its own norm step.
+// host round trip between them.  The cycle's last norm fold ends every
+// cycle: it records the cycle's norms in hist and applies the stop rule of
+// the host's norm step to them; once it stops, it sets DevLevel::stop of
+// the instance's levels and every later kernel of the batch returns at
+// once.  The host then replays hist through its own norm step.
 constexpr int kGateHist = 64;
 struct CycleGate {
   int stop;                    // set when the stop rule holds (or hist is full)
@@ -217,11 +217,19 @@ inline unsigned gm_magic(unsigned d, int* sh) {
   *sh = s1 | (s2 << 8);
   return (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
 }
-void launch_gate_check(CycleGate* g, const double* out, const double* outc, DevLevel* L, int nL,
-                       int inst, hipStream_t s);
+// The end of a gated cycle, run by the cycle's last norm fold: g (nullptr:
+// no gate), the cycle's norms (out: 4 values; outc: the 2 relmax values of
+// an FP64 solve that reports them), and the instance's level descriptors.
+struct GateArgs {
+  CycleGate* g;
+  const double* out;
+  const double* outc;
+  DevLevel* Lt;
+  int nL, inst;
+};
 void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s);
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum,
-                     double* out, int ostride, hipStream_t s);
+                     double* out, int ostride, hipStream_t s, const GateArgs* gate = nullptr);
 
 // Conversions between the reference's ghosted Fortran arrays (one pointer
 // per item in 'ptr') and the FP64 finest level.

@@ -1148,7 +1148,8 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     s += cz[nz + k] * vp;
     const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
     const double res = cur.fb - (s + diag * vc);
-    if constexpr (SHIFT) st_row<double>(f + pbk, dc, cur.fb);
+    // the shifted b and the FP32 defect are read by later kernels only
+    if constexpr (SHIFT) st_row_nt<double>(f + pbk, dc, cur.fb);
     if constexpr (MODE == kCorrDefect32) {
       // the new p is read in the next cycle and the FP32 defect by the next
       // V-cycle's first sweep, both after many more bytes than the L2 holds
@@ -1158,7 +1159,7 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
       v[1] = amax(v[1], (double)ec);
       v[2] = amax(v[2], vc);
     } else {
-      if constexpr (MODE == kDefect32) st_row<float>(r32 + pbk, sc, (float)res);
+      if constexpr (MODE == kDefect32) st_row_nt<float>(r32 + pbk, sc, (float)res);
       v[0] = amax(v[0], res);
       v[1] = amax(v[1], cur.fb);
     }
@@ -1239,31 +1240,68 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
   block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
 }
 
-// One block per item: thread t folds partials b0+t, b0+t+256, ... in order,
-// then a fixed-shape tree combines the threads (deterministic).
-__global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
-                                                     int nvals, int op_sum, double* __restrict__ out,
-                                                     int ostride) {
-  __shared__ double sm[kBlock / 64];
+// End of a gated cycle (one thread): record the cycle's norms and apply the
+// stop rule of the host's norm step (Context::post_norms) to them, with the
+// same double operations, so the host's replay reaches the same decision.
+// On a stop it sets the stop word of every level descriptor of instance
+// inst: the kernels of the later cycles of the batch read it with the rest
+// of their descriptor and return at once.
+__device__ void gate_check(const GateArgs& ga, const double* out) {
+  CycleGate* g = ga.g;
+  if (g->stop) return;
+  const int n = g->count;
+  double* h = g->hist[n];
+  for (int q = 0; q < 4; ++q) h[q] = out[q];
+  h[4] = g->report ? ga.outc[0] : 0.0;
+  h[5] = g->report ? ga.outc[1] : 0.0;
+  const double rmax = out[0];
+  double relmax;
+  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
+  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
+  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
+  g->count = n + 1;
+  const int iters = g->iters0 + n + 1;
+  int stop = 0;
+  if (!isfinite(rmax)) {
+    stop = 1;
+  } else {
+    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
+    if (crit <= g->tol || iters >= g->maxcy) stop = 1;
+  }
+  if (n + 1 >= kGateHist) stop = 1;   // history full: the host runs the next batch
+  g->stop = stop;
+  if (stop)
+    for (int l = 0; l < ga.nL; ++l)
+      if (ga.Lt[l].inst == ga.inst) ga.Lt[l].stop = 1;
+}
+
+// One block per item: thread t folds partials b0+t, b0+t+kFinBlock, ... in
+// order, then a fixed-shape tree combines the threads (deterministic).  With
+// a gate (a gated cycle's last fold, one item), thread 0 then ends the cycle.
+constexpr int kFinBlock = 1024;
+__global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
+                                                        int nvals, int op_sum, double* __restrict__ out,
+                                                        int ostride, GateArgs ga) {
+  __shared__ double sm[kFinBlock / 64];
   const int t = blockIdx.x;
   const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // one pass over the partials for every value (nvals <= 4), kFinU
   // partials per thread and round: their loads (clamped into the item, so
   // unconditional) are issued before they are folded, in order
-  constexpr int kFinU = 8;
+  constexpr int kFinU = 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = b0 + threadIdx.x; b < b1; b += kBlock * kFinU) {
+  for (int b = b0 + threadIdx.x; b < b1; b += kFinBlock * kFinU) {
     double x[kFinU][4];
 #pragma unroll
     for (int r = 0; r < kFinU; ++r) {
-      const int bb = min(b + r * kBlock, b1 - 1);
+      const int bb = min(b + r * kFinBlock, b1 - 1);
 #pragma unroll
       for (int q = 0; q < 4; ++q) x[r][q] = partial[(size_t)bb * nvals + min(q, nvals - 1)];
     }
 #pragma unroll
     for (int r = 0; r < kFinU; ++r) {
-      if (b + r * kBlock < b1) {
+      if (b + r * kFinBlock < b1) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           if (q < nvals) acc[q] = op_sum ? acc[q] + x[r][q] : nmax(acc[q], x[r][q]);
@@ -1277,11 +1315,12 @@ __global__ __launch_bounds__(kBlock) void k_finalize(LaunchList ll, const double
     __syncthreads();
     if (threadIdx.x == 0) {
       double r = sm[0];
-      for (int i = 1; i < kBlock / 64; ++i) r = op_sum ? r + sm[i] : nmax(r, sm[i]);
+      for (int i = 1; i < kFinBlock / 64; ++i) r = op_sum ? r + sm[i] : nmax(r, sm[i]);
       out[(size_t)t * ostride + q] = r;
     }
     __syncthreads();
   }
+  if (ga.g && t == 0 && threadIdx.x == 0) gate_check(ga, ga.out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1620,42 +1659,6 @@ inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) /
 
 }  // namespace
 
-// End of a gated cycle (one thread): record the cycle's norms and apply the
-// stop rule of the host's norm step (Context::post_norms) to them, with the
-// same double operations, so the host's replay reaches the same decision.
-// On a stop it sets the stop word of every level descriptor of instance
-// inst: the kernels of the later cycles of the batch read it with the rest
-// of their descriptor and return at once.
-__global__ void k_gate_check(CycleGate* __restrict__ g, const double* __restrict__ out,
-                             const double* __restrict__ outc, DevLevel* __restrict__ Lt, int nL,
-                             int inst) {
-  if (threadIdx.x != 0 || g->stop) return;
-  const int n = g->count;
-  double* h = g->hist[n];
-  for (int q = 0; q < 4; ++q) h[q] = out[q];
-  h[4] = g->report ? outc[0] : 0.0;
-  h[5] = g->report ? outc[1] : 0.0;
-  const double rmax = out[0];
-  double relmax;
-  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
-  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
-  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
-  g->count = n + 1;
-  const int iters = g->iters0 + n + 1;
-  int stop = 0;
-  if (!isfinite(rmax)) {
-    stop = 1;
-  } else {
-    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
-    if (crit <= g->tol || iters >= g->maxcy) stop = 1;
-  }
-  if (n + 1 >= kGateHist) stop = 1;   // history full: the host runs the next batch
-  g->stop = stop;
-  if (stop)
-    for (int l = 0; l < nL; ++l)
-      if (Lt[l].inst == inst) Lt[l].stop = 1;
-}
-
 // Clears the stop words of instance inst after a batch.
 __global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, int inst) {
   for (int l = threadIdx.x; l < nL; l += blockDim.x)
@@ -1817,16 +1820,14 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
 }
-void launch_gate_check(CycleGate* g, const double* out, const double* outc, DevLevel* L, int nL,
-                       int inst, hipStream_t s) {
-  hipLaunchKernelGGL(k_gate_check, dim3(1), dim3(64), 0, s, g, out, outc, L, nL, inst);
-}
 void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s) {
   hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(64), 0, s, L, nL, inst);
 }
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
-                     int ostride, hipStream_t s) {
-  LAUNCH(k_finalize, ll.nitems, kBlock, ll, partial, nvals, op_sum, out, ostride);
+                     int ostride, hipStream_t s, const GateArgs* gate) {
+  GateArgs ga{};
+  if (gate) ga = *gate;
+  LAUNCH(k_finalize, ll.nitems, kFinBlock, ll, partial, nvals, op_sum, out, ostride, ga);
 }
 void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr, int which,
                          int add_fold, hipStream_t s) {

@@ -532,14 +532,17 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S, bool gated) {
   vcycle(S, fp64);
   auto items = march_items(*this, S, true);
   const int nv = 4;
+  // gated (one instance): the cycle's last norm fold also ends the cycle
+  const GateArgs ga{d_gate, d_out, d_out + kOutRegion * 2, dL, (int)hL.size(), S[0]->id};
+  const GateArgs* gate_res = (gated && !relmax) ? &ga : nullptr;
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
     if (fp64) {
       launch_residual_max(dL, ll, part, stream);
-      launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
+      launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream, gate_res);
     } else {
       launch_correct_defect32(dL, ll, part, stream);
-      launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream);
+      launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream, gate_res);
     }
   });
   if (relmax) {
@@ -547,13 +550,11 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S, bool gated) {
     for_lists(citems, [&](const LaunchList& ll, int ioff, int boff) {
       double* part = d_partial + (size_t)boff * nv;
       launch_change_max(dL, ll, part, stream);
-      launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream);
+      launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream,
+                      gated ? &ga : nullptr);
     });
   }
-  if (gated) {
-    launch_gate_check(d_gate, d_out, d_out + kOutRegion * 2, dL, (int)hL.size(), S[0]->id, stream);
-    return AMG_OK;
-  }
+  if (gated) return AMG_OK;
   (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
   if (relmax)
     (void)hipMemcpyAsync(h_out + kOutRegion * 2, d_out + kOutRegion * 2, sizeof(double) * S.size() * 2,
@@ -712,8 +713,8 @@ int Context::finish(const std::vector<Instance*>& S) {
 // The cycles of one instance in gated batches: a batch enqueues the cycle
 // graphs of as many cycles as the instance's previous solve took (one when
 // there is none) without a host round trip between them; each cycle ends
-// with k_gate_check, and once it stops the batch, the rest of the batch's
-// kernels return at once.  After the batch the host replays the recorded
+// with a gate check (in its last norm fold), and once it stops the batch,
+// the rest of the batch's kernels return at once.  After the batch the host replays the recorded
 // norms through post_norms, which must reach the same decision at the same
 // cycle.  The graphs (FP64; mixed for each parity of the current FP64
 // buffer) are captured with the gate in their launch lists.  When capture
