@@ -44,6 +44,7 @@ struct DevLevel {
   const double* rho;         // rho(z) interior, nz values
   const double* bcfold;      // [x(nx) | y(ny) | z(nz)] boundary-value RHS shifts (finest only)
   const double* minvT;       // coarsest level: transposed dense (pseudo-)inverse, ncell^2
+  const double* minv;        // the same, row-major, when the coarse solve is its own kernel
   void* u;                   // solution / correction (chequer layout)
   void* u2;                  // FP64 finest: ping-pong partner of u
   void* f;                   // right-hand side (chequer layout)
@@ -118,6 +119,11 @@ constexpr int kTailMaxLevels = 16;
 enum { kStepZero = 0, kStepHalf = 1, kStepRestrict = 2, kStepCoarse = 3, kStepProlong = 4 };
 constexpr int kTailMaxSteps = 1024;
 constexpr size_t kTailMinvLds = 16384;   // largest coarsest inverse staged in LDS (bytes)
+// A coarsest level of at least kSplitCells cells (an inverse of 512 KiB in
+// FP64) is solved by its own kernel across the device: the tail runs in two
+// phases around it (one workgroup reading the whole inverse is bound by its
+// memory latency, about 65 GB/s).
+constexpr int kSplitCells = 256;
 struct DevTail {
   int nlev;                       // tail levels, the last is the coarsest
   int copy_u_in;                  // the first tail level's u is read (not zero)
@@ -171,8 +177,14 @@ void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int pa
 // variant: the kernel's column length class (tail_nz_class) and whether
 // every tail transfer coarsens 2:1 in all three directions.
 int tail_nz_class(bool fp64, int nz_max, bool zline);
+// phase 0: the whole sub-cycle; 1: down to the coarsest level, then every
+// tail level's u and f are saved to their global arrays; 2: those restored,
+// up from the coarsest level (whose u the coarse kernel wrote).
 void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
-                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s);
+                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s,
+                 int phase = 0);
+// u = M f on coarsest levels (row-major inverse), one wave per row.
+void launch_coarse_wide(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
 size_t tail_lds_limit();
 
 // ---- FP64 finest-level kernels (items index FP64 finest entries) -------------

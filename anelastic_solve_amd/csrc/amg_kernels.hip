@@ -895,6 +895,21 @@ __device__ __forceinline__ void coarse_row(const DevLevel& L, UP u, FP f, MP Mt,
   const auto M = Mt + r;
   double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int s = 0;
+  // 16 matrix loads in flight per row: a large coarsest level (an 8 MB
+  // inverse at 1024 cells) is read by one workgroup, and with fewer loads
+  // per thread its rate is set by the memory latency.  Element s+q adds to
+  // acc[(s+q) & 7] in increasing s, as in the 8-wide loop below.
+  for (; s + 16 <= n; s += 16) {
+    double m[16];
+    T fv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      m[q] = M[(s + q) * n];
+      fv[q] = f[s + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q & 7] += m[q] * (double)fv[q];
+  }
   for (; s + 8 <= n; s += 8) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] += M[(s + q) * n] * (double)f[s + q];
@@ -942,10 +957,13 @@ __device__ __forceinline__ void stage_copy(DST dst, SRC src, int n, int t, int n
 template <class T, int NZ, bool ALL222>
 // NZ = 0 is the point-smoother variant: no line-solve code (whose column
 // registers would cap the workgroup at 512 lanes), and 1024 lanes.
+// phase 0: the whole step list; 1: the steps before the coarse solve, then
+// every level's u and f saved to its global arrays; 2: those restored, the
+// steps after the coarse solve (run by k_coarse_wide in between).
 __global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) void k_tail(const DevLevel* __restrict__ Lt,
                                                           const DevXfer* __restrict__ X,
                                                           const DevTail* __restrict__ Tt,
-                                                          LaunchList ll) {
+                                                          LaunchList ll, int phase) {
   extern __shared__ unsigned char lds_raw[];
   __shared__ DevLevel Ls[kTailMaxLevels];
   const DevTail& tl = Tt[ll.items[blockIdx.x]];
@@ -972,12 +990,24 @@ __global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) v
     stage_copy<T>((sp<kS, T>)(base + tl.lds_minv), gp(G.minvT), G.ncell * G.ncell, tid, nthr);
   }
   const DevLevel& G0 = Lt[tl.lev[0]];
-  stage_copy<T>(Fv(0), gfield<const T>((const void*)G0.f), G0.ncell, tid, nthr);
-  if (tl.copy_u_in) stage_copy<T>(U(0), gfield<const T>((const void*)G0.u), G0.ncell, tid, nthr);
+  if (phase == 2) {
+    for (int d = 0; d < nl; ++d) {
+      const DevLevel& G = Lt[tl.lev[d]];
+      stage_copy<T>(U(d), gfield<const T>((const void*)G.u), G.ncell, tid, nthr);
+      if (d + 1 < nl) stage_copy<T>(Fv(d), gfield<const T>((const void*)G.f), G.ncell, tid, nthr);
+    }
+  } else {
+    stage_copy<T>(Fv(0), gfield<const T>((const void*)G0.f), G0.ncell, tid, nthr);
+    if (tl.copy_u_in) stage_copy<T>(U(0), gfield<const T>((const void*)G0.u), G0.ncell, tid, nthr);
+  }
   __syncthreads();
+  // the coarse solve's place in the step list (phases 1 and 2 stop or start there)
+  int ci = 0;
+  while (ci < tl.nsteps && (tl.steps[ci] & 7) != kStepCoarse) ++ci;
+  const int st0 = phase == 2 ? ci + 1 : 0, st1 = phase == 1 ? ci : tl.nsteps;
   // one routine per step type, so that each element routine is instantiated
   // once (the kernel stays small enough for the instruction cache)
-  for (int st = 0; st < tl.nsteps; ++st) {
+  for (int st = st0; st < st1; ++st) {
     const int code = tl.steps[st];
     const int type = code & 7, d = (code >> 3) & 15, par = (code >> 7) & 1, zg = (code >> 8) & 3;
     const DevLevel& L = Ls[d];
@@ -1019,9 +1049,55 @@ __global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) v
     }
     __syncthreads();
   }
+  if (phase == 1) {
+    for (int d = 0; d < nl; ++d) {
+      const DevLevel& G = Lt[tl.lev[d]];
+      const sp<kG, T> gu = gfield<T>(G.u), gf = gfield<T>(G.f);
+      const sp<kS, T> lu = U(d), lf = Fv(d);
+      for (int e = tid; e < G.ncell; e += nthr) {
+        gu[e] = lu[e];
+        gf[e] = lf[e];
+      }
+    }
+    return;
+  }
   const sp<kG, T> gu = gfield<T>(G0.u);
   const sp<kS, T> lu = U(0);
   for (int e = tid; e < G0.ncell; e += nthr) gu[e] = lu[e];
+}
+
+// Coarsest level of a split tail: u = M f with the row-major (pseudo-)inverse,
+// one wave per row (FP64 partial sums over the lane's columns in a fixed
+// order, then a fixed butterfly), so a large inverse is read by the whole
+// device rather than by one workgroup.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_coarse_wide(const DevLevel* __restrict__ Lt, LaunchList ll) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
+  const int n = L.ncell;
+  const int r = (b - ll.prefix[t]) * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= n) return;
+  const sp<kG, const double> M = gp(L.minv) + (size_t)r * n;
+  const sp<kG, const T> f = gfield<const T>((const void*)L.f);
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int s = lane;
+  for (; s + 7 * 64 < n; s += 8 * 64) {
+    double m[8], fv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      m[q] = M[s + q * 64];
+      fv[q] = (double)f[s + q * 64];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] += m[q] * fv[q];
+  }
+  for (int q = 0; s < n; s += 64, ++q) acc[q & 7] += M[s] * (double)f[s];
+  double sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  sum = wave_sum(sum);
+  if (lane == 0) gfield<T>(L.u)[r] = (T)sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -1740,13 +1816,13 @@ int tail_nz_class(bool fp64, int nz_max, bool zline) {
 
 #define TAIL_KERNEL(T, NZ, F) (k_tail<T, NZ, F>)
 void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const DevXfer* X,
-                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s) {
+                 const DevTail* T, const LaunchList& ll, size_t lds_bytes, hipStream_t s, int phase) {
   const dim3 grd(ll.nitems);
-#define GO(TT, NZ)                                                                         \
-  do {                                                                                     \
-    const dim3 blk(64, NZ == 0 ? kTailWavesPoint : kTailWaves);                            \
-    if (all222) hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, true), grd, blk, lds_bytes, s, L, X, T, ll);  \
-    else hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, false), grd, blk, lds_bytes, s, L, X, T, ll);        \
+#define GO(TT, NZ)                                                                                   \
+  do {                                                                                               \
+    const dim3 blk(64, NZ == 0 ? kTailWavesPoint : kTailWaves);                                      \
+    if (all222) hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, true), grd, blk, lds_bytes, s, L, X, T, ll, phase); \
+    else hipLaunchKernelGGL(TAIL_KERNEL(TT, NZ, false), grd, blk, lds_bytes, s, L, X, T, ll, phase);       \
   } while (0)
   if (fp64) {
     if (nz_class == 0) GO(double, 0); else if (nz_class <= 8) GO(double, 8); else GO(double, 16);
@@ -1757,6 +1833,11 @@ void launch_tail(bool fp64, int nz_class, bool all222, const DevLevel* L, const 
     else GO(float, 32);
   }
 #undef GO
+}
+
+void launch_coarse_wide(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s) {
+  if (fp64) LAUNCH(k_coarse_wide<double>, ll.nblocks, kBlock, L, ll);
+  else LAUNCH(k_coarse_wide<float>, ll.nblocks, kBlock, L, ll);
 }
 
 size_t tail_lds_limit() {

@@ -198,6 +198,16 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
       if (!mv) return fail();
       AMG_HIP(hipMemcpy(mv, I->H.minvT.data(), I->H.minvT.size() * 8, hipMemcpyHostToDevice));
       d.minvT = mv;
+      if (d.ncell >= kSplitCells) {
+        const size_t n = (size_t)d.ncell;
+        std::vector<double> m(n * n);
+        for (size_t r = 0; r < n; ++r)
+          for (size_t c = 0; c < n; ++c) m[r * n + c] = I->H.minvT[c * n + r];
+        double* mr = (double*)I->mem.alloc(n * n * 8);
+        if (!mr) return fail();
+        AMG_HIP(hipMemcpy(mr, m.data(), n * n * 8, hipMemcpyHostToDevice));
+        d.minv = mr;
+      }
     }
     // fields of the V-cycle precision
     d.u = I->mem.alloc((size_t)d.ncell * w);
@@ -339,6 +349,7 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     }
     I->tail_nzc = tail_nz_class(I->fp64, nzmax, cfg.smoother == AMG_SMOOTHER_LJ);
     I->tail_222 = all222;
+    I->tail_split = tl.lds_minv < 0 && (int)nco >= kSplitCells;
     I->tail = (int)hT.size();
     I->tail_lds = off;
     hT.push_back(tl);
@@ -465,19 +476,37 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
       launch_restrict(fp64, L, dX, ll, stream);
     });
   }
-  // the coarse tails, one launch per kernel variant
+  // the coarse tails, one launch per kernel variant; a split tail runs down,
+  // then the coarse solve across the device, then up
   for (int nzc : {0, 8, 16, 32}) {
     for (int a222 = 0; a222 < 2; ++a222) {
-      std::vector<Item> items;
-      size_t lds = 0;
-      for (Instance* I : S) {
-        if (I->tail_nzc != nzc || (int)I->tail_222 != a222) continue;
-        items.push_back({I->tail, 1});
-        lds = std::max(lds, I->tail_lds);
+      for (int split = 0; split < 2; ++split) {
+        std::vector<Item> items, coarse;
+        size_t lds = 0;
+        for (Instance* I : S) {
+          if (I->tail_nzc != nzc || (int)I->tail_222 != a222 || (int)I->tail_split != split) continue;
+          items.push_back({I->tail, 1});
+          lds = std::max(lds, I->tail_lds);
+          const int lc = I->lev.back();
+          coarse.push_back({lc, cdiv(hL[lc].ncell, kBlock / 64)});
+        }
+        if (items.empty()) continue;
+        if (!split) {
+          for_lists(items, [&](const LaunchList& ll, int, int) {
+            launch_tail(fp64, nzc, a222 != 0, L, dX, dT, ll, lds, stream);
+          });
+          continue;
+        }
+        for_lists(items, [&](const LaunchList& ll, int, int) {
+          launch_tail(fp64, nzc, a222 != 0, L, dX, dT, ll, lds, stream, 1);
+        });
+        for_lists(coarse, [&](const LaunchList& ll, int, int) {
+          launch_coarse_wide(fp64, L, ll, stream);
+        });
+        for_lists(items, [&](const LaunchList& ll, int, int) {
+          launch_tail(fp64, nzc, a222 != 0, L, dX, dT, ll, lds, stream, 2);
+        });
       }
-      for_lists(items, [&](const LaunchList& ll, int, int) {
-        launch_tail(fp64, nzc, a222 != 0, L, dX, dT, ll, lds, stream);
-      });
     }
   }
   for (int dpt = maxL0 - 1; dpt >= 0; --dpt) {

@@ -55,6 +55,7 @@ struct Instance {
   size_t tail_lds = 0;           // dynamic LDS bytes of the tail
   int tail_nzc = 16;             // tail kernel variant: column length class
   bool tail_222 = true;          //   and uniform 2:1 transfers
+  bool tail_split = false;       // the coarse solve is its own kernel (large coarsest level)
   int fine[2] = {-1, -1};        // FP64 finest entries (u = p_cur for cur = 0/1)
   int cur = 0;                   // which fine entry holds the current pressure
   std::vector<int> pre, post;
