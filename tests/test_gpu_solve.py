@@ -267,3 +267,32 @@ def test_gated_batches_equal_fresh_solves(cuda, precision):
             assert it == fresh[name][1] and rr == fresh[name][2], (order, name, it, fresh[name][1])
             assert torch.equal(p, fresh[name][0]), (order, name)
         s.close()
+
+
+def test_split_coarse_tail(cuda):
+    """A coarsest level of 256 cells or more is solved by its own kernel
+    across the device, between two phases of the coarse tail.  A batch that
+    mixes such instances (coarsest 16x16x4 and 8x8x4) with one whose coarsest
+    inverse stays inside the tail (4x4x4) gives bitwise the solves alone,
+    and a split solve reaches the oracle's direct solution."""
+    torch = cuda
+    from anelastic_solve_amd import BatchSolver, PressureSolver, box_grid, default_config, workloads
+    specs = [((32, 32, 8), 1.0), ((16, 16, 8), 0.5), ((8, 8, 8), 0.0)]
+    cfg = lambda: default_config(nlevels=(2, 2, 2), tol=1e-12)
+    grids = [box_grid(*n, beta=b) for n, b in specs]
+    rhs = [torch.from_numpy(workloads.psource_host(*workloads.velocity_white(g, 40 + q), g)).cuda()
+           for q, g in enumerate(grids)]
+    bs = BatchSolver(grids, cfg())
+    phis = [torch.zeros(g.shape, dtype=torch.float64, device="cuda") for g in grids]
+    its, rr = bs.solve(phis, rhs)
+    for q, g in enumerate(grids):
+        s = PressureSolver(g, cfg())
+        assert [tuple(l) for l in s.levels][-1] == tuple(n // 2 for n in specs[q][0])
+        p = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
+        it, _ = s.mg_solver(p, rhs[q])
+        assert it == its[q] and rr[q] <= 1e-12
+        assert torch.equal(p, phis[q]), q
+        s.close()
+    g, b = grids[0], rhs[0].cpu().numpy()
+    x, _ = _direct(g, b, tuple(cfg().tbc), None)     # the finest operator does not depend on the levels
+    assert relerr(phis[0].cpu().numpy()[1:-1, 1:-1, 1:-1], x) <= 1e-9
