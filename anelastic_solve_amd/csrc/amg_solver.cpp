@@ -95,10 +95,17 @@ int Context::init(const amg_config& c) {
   return AMG_OK;
 }
 
+void Context::drop_graphs() {
+  for (auto& g : gexec)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  for (auto& row : gchain)
+    for (auto& g : row)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
 Context::~Context() {
   if (stream) (void)hipStreamSynchronize(stream);
-  for (auto& g : gexec)
-    if (g) (void)hipGraphExecDestroy(g);
+  drop_graphs();
   for (Instance* I : inst) delete I;
   inst.clear();
   if (dL) (void)hipFree(dL);
@@ -407,8 +414,7 @@ int Context::upload_tables() {
     cap_partial = need * 2;
     AMG_HIP(hipMalloc((void**)&d_partial, cap_partial * sizeof(double)));
   }
-  for (auto& g : gexec)
-    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+  drop_graphs();
   return AMG_OK;
 }
 
@@ -739,39 +745,50 @@ int Context::finish(const std::vector<Instance*>& S) {
   return AMG_OK;
 }
 
-// The cycles of one instance in gated batches: a batch enqueues the cycle
-// graphs of as many cycles as the instance's previous solve took (one when
-// there is none) without a host round trip between them; each cycle ends
-// with a gate check (in its last norm fold), and once it stops the batch,
-// the rest of the batch's kernels return at once.  After the batch the host replays the recorded
-// norms through post_norms, which must reach the same decision at the same
-// cycle.  The graphs (FP64; mixed for each parity of the current FP64
-// buffer) are captured with the gate in their launch lists.  When capture
-// is unavailable the caller's ungated loop takes over.
+// The cycles of one instance in gated batches: a batch enqueues as many
+// cycles as the instance's previous solve took (one when there is none)
+// without a host round trip between them; each cycle ends with a gate
+// check (in its last norm fold), and once it stops the batch, the rest of
+// the batch's kernels return at once.  After the batch the host replays the
+// recorded norms through post_norms, which must reach the same decision at
+// the same cycle.  A batch runs as graphs of 4, 2 and 1 consecutive cycles
+// (the fewest graph launches: each graph boundary costs the device several
+// microseconds), captured with the gate in their launch lists, per chain
+// length and, in mixed precision, per parity of the current FP64 buffer at
+// the chain's start.  When capture is unavailable the caller's ungated loop
+// takes over.
 int Context::gated_cycles(Instance* I, double* bytes) {
   const std::vector<Instance*> A{I};
   while (!I->done && I->iters < cfg.maxcy) {
     int batch = I->last_iters > I->iters ? I->last_iters - I->iters : 1;
     batch = std::min(batch, std::min(kGateHist, cfg.maxcy - I->iters));
-    // the graphs this batch needs
-    for (int b = 0; b < std::min(batch, 2); ++b) {
-      const int par = (I->cur + b) & 1;
-      const int gi = I->fp64 ? 0 : 1 + par;
-      if (gexec[gi]) continue;
+    // the batch as chains (length index, graph index), longest first
+    std::vector<std::pair<int, int>> plan;
+    for (int left = batch, par = I->cur; left > 0;) {
+      const int q = left >= 4 ? 2 : left >= 2 ? 1 : 0;
+      plan.push_back({q, I->fp64 ? 0 : 1 + par});
+      left -= kChainLen[q];
+      if (!I->fp64) par = (par + kChainLen[q]) & 1;
+    }
+    for (const auto& [q, gi] : plan) {
+      if (gchain[q][gi]) continue;
       const int cur0 = I->cur;
-      I->cur = par;
+      if (!I->fp64) I->cur = gi - 1;
       hipGraph_t g = nullptr;
       bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
-        cycle_and_norms(A, true);
+        for (int c = 0; c < kChainLen[q]; ++c) {
+          cycle_and_norms(A, true);
+          if (!I->fp64) I->cur ^= 1;     // as post_norms does after a mixed cycle
+        }
         ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
       }
-      if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
+      if (ok) ok = hipGraphInstantiate(&gchain[q][gi], g, nullptr, nullptr, 0) == hipSuccess;
       if (g) (void)hipGraphDestroy(g);
       I->cur = cur0;
       if (!ok) {
         graph_ok = false;
-        gexec[gi] = nullptr;
+        gchain[q][gi] = nullptr;
         (void)hipGetLastError();
         return AMG_OK;
       }
@@ -788,8 +805,7 @@ int Context::gated_cycles(Instance* I, double* bytes) {
     h.bnorm = I->bnorm;
     h.tol = cfg.tol;
     AMG_HIP(hipMemcpyAsync(d_gate, h_gate, offsetof(CycleGate, hist), hipMemcpyHostToDevice, stream));
-    for (int b = 0; b < batch; ++b)
-      AMG_HIP(hipGraphLaunch(gexec[I->fp64 ? 0 : 1 + ((I->cur + b) & 1)], stream));
+    for (const auto& [q, gi] : plan) AMG_HIP(hipGraphLaunch(gchain[q][gi], stream));
     AMG_HIP(hipMemcpyAsync(h_gate, d_gate, sizeof(CycleGate), hipMemcpyDeviceToHost, stream));
     launch_gate_reset(dL, (int)hL.size(), I->id, stream);
     AMG_HIP(hipStreamSynchronize(stream));

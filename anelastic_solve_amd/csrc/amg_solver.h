@@ -95,6 +95,10 @@ struct Context {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // single-instance graphs: [0] fp64 cycle, [1]/[2] mixed cycle for cur = 0/1
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  // gated single-instance graphs of kChainLen[q] cycles, indexed as gexec
+  static constexpr int kChainLen[3] = {1, 2, 4};
+  hipGraphExec_t gchain[3][3] = {};
+  void drop_graphs();
   bool graph_ok = true;
 
   int init(const amg_config& c);
