@@ -165,6 +165,24 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
                   int zero_guess, int segs, hipStream_t s);
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                      hipStream_t s);
+// Residual + restriction marching in z, for a transfer with factor 2 in every
+// direction and fine sizes twice the coarse ones, on coarse levels of at
+// least kRestrictMinCols columns (restrict222_ok): a thread walks
+// kRestrictK coarse planes, which on a smaller level leaves too few threads
+// (there the per-cell kernel is faster).  Blocks per item =
+// restrict222_blocks(coarse).  Same result as launch_restrict.
+constexpr int kRestrictK = 8;            // coarse planes per thread
+constexpr int kRestrictMinCols = 16384;  // e.g. a 128^2 coarse plane
+inline bool restrict222_ok(const DevLevel& F, const DevLevel& C, const DevXfer& x) {
+  return x.fx == 2 && x.fy == 2 && x.fz == 2 && F.nx == 2 * C.nx && F.ny == 2 * C.ny &&
+         F.nz == 2 * C.nz && F.nx == 2 * F.hx && C.nx == 2 * C.hx &&
+         (long long)C.nx * C.ny >= kRestrictMinCols;
+}
+inline int restrict222_blocks(const DevLevel& C) {
+  return (int)(((long long)C.nx * C.ny + kBlock - 1) / kBlock) * ((C.nz + kRestrictK - 1) / kRestrictK);
+}
+void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                        hipStream_t s);
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s);
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);

@@ -734,6 +734,134 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
                     coefs<T>(F), gp(F.vol), C, gfield<T>(C.f), x, e);
 }
 
+// Residual + restriction of a transfer with factor 2 in every direction and
+// fine sizes exactly twice the coarse ones, marching in z.  A thread owns
+// coarse column (I, J) over a slab of kRestrictK coarse planes: fine
+// columns i in {2I, 2I+1}, j in {2J, 2J+1}.  It walks their fine planes
+// upward with the centre values of the planes below, at and above in
+// registers; per coarse cell it loads the centres of the two planes above,
+// b of its two planes and the eight in-plane neighbours outside its 2x2
+// block (16 loads per fine plane, against 32 for restrict_elem).  The
+// neighbours inside the block are its own centres, at the indices
+// neighbours() gives, and the outside ones take its wraps.  Each fine
+// residual is formed with offsum/diag_at as in restrict_elem, and the coarse
+// value sums its eight in the same order (z, then y, then x), so the result
+// is restrict_elem's.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restrict__ Lt,
+                                                        const DevXfer* __restrict__ X,
+                                                        LaunchList ll) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& C = Lt[x.coarse];
+  const DevLevel& F = Lt[x.fine];
+  if (level_off(F)) return;
+  const int cols = C.nx * C.ny;
+  const int ncb = (cols + kBlock - 1) / kBlock;
+  const int lb = b - ll.prefix[t];
+  const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
+  if (col >= cols) return;
+  const int J = col / C.nx, I = col - J * C.nx;
+  const int K0 = bk * kRestrictK, K1 = min(K0 + kRestrictK, C.nz);
+  const auto u = gfield<const T>((const void*)F.u);
+  const auto f = gfield<const T>((const void*)F.f);
+  const auto co = coefs<T>(F);
+  const auto vx = gp(F.vol);
+  const auto vy = vx + F.nx;
+  const auto vz = vy + F.ny;
+  const int hx = F.hx, ny = F.ny, nz = F.nz;
+  const int plane = 2 * hx * ny;
+  // in-plane offsets: own cells o[cy][cx], outside neighbours
+  int o[2][2], oxm[2], oxp[2], oym[2], oyp[2];
+#pragma unroll
+  for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx) {
+      const int i = 2 * I + cx, j = 2 * J + cy;
+      o[cy][cx] = cidx(i >> 1, j, 0, (i + j) & 1, hx, ny);
+    }
+#pragma unroll
+  for (int cy = 0; cy < 2; ++cy) {
+    const int j = 2 * J + cy;
+    // x- of i = 2I (r = 0: m - 1, wrapped) and x+ of i = 2I+1 (r = 1: m + 1, wrapped)
+    const int ml = I == 0 ? hx - 1 : I - 1, mr = I + 1 >= hx ? 0 : I + 1;
+    oxm[cy] = cidx(0, j, 0, ((2 * I + j) & 1) ^ 1, hx, ny) + ml;
+    oxp[cy] = cidx(0, j, 0, ((2 * I + 1 + j) & 1) ^ 1, hx, ny) + mr;
+  }
+#pragma unroll
+  for (int cx = 0; cx < 2; ++cx) {
+    const int i = 2 * I + cx;
+    const int j0 = 2 * J, j1 = 2 * J + 1;
+    const int jm = j0 == 0 ? ny - 1 : j0 - 1, jp = j1 == ny - 1 ? 0 : j1 + 1;
+    oym[cx] = cidx(i >> 1, jm, 0, ((i + j0) & 1) ^ 1, hx, ny);
+    oyp[cx] = cidx(i >> 1, jp, 0, ((i + j1) & 1) ^ 1, hx, ny);
+  }
+  auto ld4 = [&](int k, T (&v)[2][2]) {
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx) v[cy][cx] = u[k * plane + o[cy][cx]];
+  };
+  T um[2][2], uc[2][2];
+  const int k00 = 2 * K0;
+  ld4(k00 == 0 ? nz - 1 : k00 - 1, um);
+  ld4(k00, uc);
+  for (int K = K0; K < K1; ++K) {
+    const int ka = 2 * K, kb = ka + 1, kc = kb == nz - 1 ? 0 : kb + 1;
+    // every load of the coarse cell first
+    T ua[2][2], ub[2][2], fv[2][2][2], xm[2][2], xp[2][2], ym[2][2], yp[2][2];
+    ld4(kb, ua);
+    ld4(kc, ub);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int kk = ka + h;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        fv[h][q][0] = f[kk * plane + o[q][0]];
+        fv[h][q][1] = f[kk * plane + o[q][1]];
+        xm[h][q] = u[kk * plane + oxm[q]];
+        xp[h][q] = u[kk * plane + oxp[q]];
+        ym[h][q] = u[kk * plane + oym[q]];
+        yp[h][q] = u[kk * plane + oyp[q]];
+      }
+    }
+    T acc = T(0), vs = T(0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = ka + h;
+      const T (&zm)[2][2] = h ? uc : um;
+      const T (&cc)[2][2] = h ? ua : uc;
+      const T (&zp)[2][2] = h ? ub : ua;
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy) {
+        const int j = 2 * J + cy;
+#pragma unroll
+        for (int cx = 0; cx < 2; ++cx) {
+          const int i = 2 * I + cx;
+          const T uxm = cx ? cc[cy][0] : xm[h][cy];
+          const T uxp = cx ? xp[h][cy] : cc[cy][1];
+          const T uym = cy ? cc[0][cx] : ym[h][cx];
+          const T uyp = cy ? yp[h][cx] : cc[1][cx];
+          const T s = offsum<T>(co, F, i, j, k, uxm, uxp, uym, uyp, zm[cy][cx], zp[cy][cx]);
+          const T rr = fv[h][cy][cx] - (s + diag_at<T>(co, F, i, j, k) * cc[cy][cx]);
+          const T v = T(vz[k] * vy[j] * vx[i]);
+          acc += v * rr;
+          vs += v;
+        }
+      }
+    }
+    gfield<T>(C.f)[cidx(I >> 1, J, K, (I + J) & 1, C.hx, C.ny)] = acc / vs;
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx) {
+        um[cy][cx] = ua[cy][cx];
+        uc[cy][cx] = ub[cy][cx];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Trilinear prolongation-add (mgMK/mg_cor.F90:102-159): fine u += sum of
 // 27,9,9,3,9,3,3,1 /64 of the eight nearest coarse cells; coarse ghosts are
@@ -1802,6 +1930,11 @@ void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const Launc
                      hipStream_t s) {
   if (fp64) LAUNCH(k_restrict<double>, ll.nblocks, kBlock, L, X, ll);
   else LAUNCH(k_restrict<float>, ll.nblocks, kBlock, L, X, ll);
+}
+void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
+                        hipStream_t s) {
+  if (fp64) LAUNCH(k_restrict222<double>, ll.nblocks, kBlock, L, X, ll);
+  else LAUNCH(k_restrict222<float>, ll.nblocks, kBlock, L, X, ll);
 }
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s) {

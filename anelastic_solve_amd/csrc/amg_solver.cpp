@@ -475,9 +475,17 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
         zero.push_back({I->lev[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
     for_lists(zero, [&](const LaunchList& ll, int, int) { launch_fill_zero(fp64, L, ll, stream); });
     smooth(dpt, true, zero_first);
-    std::vector<Item> items;
-    for (Instance* I : S)
-      if (dpt < I->l0) items.push_back({I->xf[dpt], cdiv(hL[I->lev[dpt + 1]].ncell, kBlock)});
+    std::vector<Item> items, items222;
+    for (Instance* I : S) {
+      if (dpt >= I->l0) continue;
+      const int xi = I->xf[dpt];
+      const DevLevel& C = hL[I->lev[dpt + 1]];
+      if (restrict222_ok(hL[I->lev[dpt]], C, hX[xi])) items222.push_back({xi, restrict222_blocks(C)});
+      else items.push_back({xi, cdiv(C.ncell, kBlock)});
+    }
+    for_lists(items222, [&](const LaunchList& ll, int, int) {
+      launch_restrict222(fp64, L, dX, ll, stream);
+    });
     for_lists(items, [&](const LaunchList& ll, int, int) {
       launch_restrict(fp64, L, dX, ll, stream);
     });
@@ -937,9 +945,11 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
       set_error("amg_run_kernel: no coarser level");
       return AMG_EINVAL;
     }
-    const int nc = which == 1 ? hL[I->lev[level + 1]].ncell
-                              : prolong_duals(d, hL[I->lev[level + 1]], hX[I->xf[level]]);
-    items.push_back({I->xf[level], cdiv(nc, kBlock)});
+    const DevLevel& c = hL[I->lev[level + 1]];
+    const bool r222 = which == 1 && restrict222_ok(d, c, hX[I->xf[level]]);
+    const int nc = which == 1 ? c.ncell : prolong_duals(d, c, hX[I->xf[level]]);
+    items.push_back({I->xf[level], r222 ? restrict222_blocks(c) : cdiv(nc, kBlock)});
+    if (r222) which = 9;   // the z-march restriction
     b = (2.0 + 0.125) * w * d.ncell;
   } else if (which == 3) {
     const DevLevel& fd = hL[I->fine[I->cur]];
@@ -973,6 +983,8 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
         launch_zline_col(fp64, dL, ll, which == 8 ? 1 : 0, 0, stream);
       } else if (which == 1) {
         launch_restrict(fp64, dL, dX, ll, stream);
+      } else if (which == 9) {
+        launch_restrict222(fp64, dL, dX, ll, stream);
       } else if (which == 2) {
         launch_prolong(fp64, dL, dX, ll, stream);
       } else if (which == 5) {

@@ -85,6 +85,33 @@ def test_half_sweeps(cuda, case, smoother, precision):
                 assert relerr(got, ref) < _tol(precision, smoother), (lvl, which)
 
 
+@pytest.mark.parametrize("tbc", [(0, 1, 0, 1, 0, 0), (2, 2, 1, 1, 2, 2), (0, 0, 0, 0, 0, 0)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_restriction_march_large_level(cuda, precision, tbc):
+    """A factor-2 restriction onto a coarse plane of 128x128 columns or more
+    runs the z-marching kernel (a thread per coarse column and slab of coarse
+    planes) instead of the per-cell one.  It agrees with the oracle's residual
+    + restriction to the same bar, across slab ends (12 coarse planes, slabs
+    of 8) and with periodic, Neumann and Dirichlet faces, periodic z
+    included."""
+    from oracle import vcycle as VC
+    s, g, levs, fac = _solver((256, 256, 24), 1.5, tbc, "LJ", precision, rho=True,
+                              nlevels=(6, 6, 3))
+    assert tuple(fac[0]) == (2, 2, 2) and tuple(levs[1]["n"]) == (128, 128, 12)
+    dt = np.float64 if precision == "fp64" else np.float32
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    rng = np.random.default_rng(300)
+    u = rng.standard_normal((24, 256, 256)).astype(dt)
+    f = rng.standard_normal((24, 256, 256)).astype(dt)
+    s.level_field(0, 0, u)
+    s.level_field(0, 1, f)
+    s.run_kernel(1, 0, 1)
+    got = s.level_field(1, 1)
+    ref = VC.restrict(VC.residual(u, f, levs[0], dt), levs[0], levs[1], fac[0], dt)
+    assert relerr(got, ref) < tol
+    s.close()
+
+
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_transfers_and_coarse_solve(cuda, case, precision):
