@@ -167,19 +167,29 @@ void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const Launc
                      hipStream_t s);
 // Residual + restriction marching in z, for a transfer with factor 2 in every
 // direction and fine sizes twice the coarse ones, on coarse levels of at
-// least kRestrictMinCols columns (restrict222_ok): a thread walks
-// kRestrictK coarse planes, which on a smaller level leaves too few threads
-// (there the per-cell kernel is faster).  Blocks per item =
-// restrict222_blocks(coarse).  Same result as launch_restrict.
-constexpr int kRestrictK = 8;            // coarse planes per thread
-constexpr int kRestrictMinCols = 16384;  // e.g. a 128^2 coarse plane
+// least kRestrictMinCells cells (restrict222_ok).  A thread walks
+// restrict222_k coarse planes: up to 8, fewer where that would leave fewer
+// than kRestrictThreads threads (a smaller level is latency-bound).  Blocks
+// per item = restrict222_blocks(coarse).  Same result as launch_restrict.
+#ifndef AMG_RESTRICT_MIN_CELLS
+#define AMG_RESTRICT_MIN_CELLS (1 << 18)
+#endif
+constexpr long long kRestrictMinCells = AMG_RESTRICT_MIN_CELLS;
+constexpr long long kRestrictThreads = 1 << 18;
+__host__ __device__ inline int restrict222_k(const DevLevel& C) {
+  const long long cells = (long long)C.nx * C.ny * C.nz;
+  int k = 8;
+  while (k > 1 && cells / k < kRestrictThreads) k >>= 1;
+  return k;
+}
 inline bool restrict222_ok(const DevLevel& F, const DevLevel& C, const DevXfer& x) {
   return x.fx == 2 && x.fy == 2 && x.fz == 2 && F.nx == 2 * C.nx && F.ny == 2 * C.ny &&
          F.nz == 2 * C.nz && F.nx == 2 * F.hx && C.nx == 2 * C.hx &&
-         (long long)C.nx * C.ny >= kRestrictMinCols;
+         (long long)C.nx * C.ny * C.nz >= kRestrictMinCells;
 }
 inline int restrict222_blocks(const DevLevel& C) {
-  return (int)(((long long)C.nx * C.ny + kBlock - 1) / kBlock) * ((C.nz + kRestrictK - 1) / kRestrictK);
+  const int kr = restrict222_k(C);
+  return (int)(((long long)C.nx * C.ny + kBlock - 1) / kBlock) * ((C.nz + kr - 1) / kr);
 }
 void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                         hipStream_t s);

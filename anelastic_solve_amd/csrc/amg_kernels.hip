@@ -736,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
 
 // Residual + restriction of a transfer with factor 2 in every direction and
 // fine sizes exactly twice the coarse ones, marching in z.  A thread owns
-// coarse column (I, J) over a slab of kRestrictK coarse planes: fine
+// coarse column (I, J) over a slab of restrict222_k coarse planes: fine
 // columns i in {2I, 2I+1}, j in {2J, 2J+1}.  It walks their fine planes
 // upward with the centre values of the planes below, at and above in
 // registers; per coarse cell it loads the centres of the two planes above,
@@ -763,7 +763,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
   const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
   if (col >= cols) return;
   const int J = col / C.nx, I = col - J * C.nx;
-  const int K0 = bk * kRestrictK, K1 = min(K0 + kRestrictK, C.nz);
+  const int kr = restrict222_k(C);
+  const int K0 = bk * kr, K1 = min(K0 + kr, C.nz);
   const auto u = gfield<const T>((const void*)F.u);
   const auto f = gfield<const T>((const void*)F.f);
   const auto co = coefs<T>(F);

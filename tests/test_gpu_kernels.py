@@ -88,21 +88,22 @@ def test_half_sweeps(cuda, case, smoother, precision):
 @pytest.mark.parametrize("tbc", [(0, 1, 0, 1, 0, 0), (2, 2, 1, 1, 2, 2), (0, 0, 0, 0, 0, 0)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_restriction_march_large_level(cuda, precision, tbc):
-    """A factor-2 restriction onto a coarse plane of 128x128 columns or more
-    runs the z-marching kernel (a thread per coarse column and slab of coarse
-    planes) instead of the per-cell one.  It agrees with the oracle's residual
-    + restriction to the same bar, across slab ends (12 coarse planes, slabs
-    of 8) and with periodic, Neumann and Dirichlet faces, periodic z
-    included."""
+    """A factor-2 restriction onto a coarse level of 2^18 cells or more runs
+    the z-marching kernel (a thread per coarse column and slab of coarse
+    planes; here 128x128x35 or x36 coarse cells, slabs of 2, the last one
+    partial with 35) instead of the per-cell one.  It agrees with the
+    oracle's residual + restriction to the same bar, with periodic, Neumann
+    and Dirichlet faces, periodic z included."""
     from oracle import vcycle as VC
-    s, g, levs, fac = _solver((256, 256, 24), 1.5, tbc, "LJ", precision, rho=True,
-                              nlevels=(6, 6, 3))
-    assert tuple(fac[0]) == (2, 2, 2) and tuple(levs[1]["n"]) == (128, 128, 12)
+    nz = 72 if tbc[4] == 0 else 70          # periodic z needs even sizes on every level
+    s, g, levs, fac = _solver((256, 256, nz), 1.5, tbc, "LJ", precision, rho=True,
+                              nlevels=(7, 7, 2))
+    assert tuple(fac[0]) == (2, 2, 2) and tuple(levs[1]["n"]) == (128, 128, nz // 2)
     dt = np.float64 if precision == "fp64" else np.float32
     tol = 1e-12 if precision == "fp64" else 2e-5
     rng = np.random.default_rng(300)
-    u = rng.standard_normal((24, 256, 256)).astype(dt)
-    f = rng.standard_normal((24, 256, 256)).astype(dt)
+    u = rng.standard_normal((nz, 256, 256)).astype(dt)
+    f = rng.standard_normal((nz, 256, 256)).astype(dt)
     s.level_field(0, 0, u)
     s.level_field(0, 1, f)
     s.run_kernel(1, 0, 1)
