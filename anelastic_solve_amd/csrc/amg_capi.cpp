@@ -14,20 +14,10 @@
 
 #include "amg.h"
 #include "amg_flow.h"
+#include "amg_handle.h"
 #include "amg_solver.h"
 
 using namespace amg;
-
-struct amg_solver_s {
-  Context ctx;
-  Instance* I = nullptr;
-  double* d_phi_stage = nullptr;   // staging for the host entry point
-  double* d_rhs_stage = nullptr;
-  double* d_div_partial = nullptr;
-  double* h_div = nullptr;
-  CnRow* d_cn = nullptr;           // rows of the vertical CN system (nz+2)
-  CnRow* h_cn = nullptr;           //   and their pinned host staging
-};
 
 struct amg_batch_s {
   Context ctx;
@@ -118,6 +108,47 @@ const std::string* get(const Ini& ini, const char* block, const char* key) {
 }
 
 }  // namespace
+
+void amg::metric_ptrs(const Instance* I, const double** rho, const double** rdxc,
+                        const double** rdyc, const double** rdzc, const double** rdxe,
+                        const double** rdye, const double** rdze) {
+  const int nx = I->nx, ny = I->ny, nz = I->nz;
+  const double* m = I->d_met;
+  *rho = m; m += nz + 2;
+  *rdxc = m; m += nx + 2;
+  *rdyc = m; m += ny + 2;
+  *rdzc = m; m += nz + 2;
+  *rdxe = m; m += nx + 2;
+  *rdye = m; m += ny + 2;
+  *rdze = m;
+}
+
+// The divergence report into `partial` (divergence_blocks * 4 values, then
+// the 4 results); copies the results to the host and fills the outputs.
+int amg::divergence_report(amg_handle h, const double* U, const double* V, const double* W,
+                             double* div, double* partial, double* avg, double* maxabs,
+                             int loc[3]) {
+  const double *rho, *rdxc, *rdyc, *rdzc, *rdxe, *rdye, *rdze;
+  metric_ptrs(h->I, &rho, &rdxc, &rdyc, &rdzc, &rdxe, &rdye, &rdze);
+  const int nx = h->I->nx, ny = h->I->ny, nz = h->I->nz;
+  const int nb = divergence_blocks(nx, ny, nz);
+  hipStream_t s = h->ctx.stream;
+  launch_divergence(U, V, W, div, rdxc, rdyc, rdzc, nx, ny, nz, partial, s);
+  launch_divergence_finalize(partial, nb, partial + 4 * (size_t)nb, s);
+  AMG_HIP(hipGetLastError());
+  double o[4];
+  AMG_HIP(hipMemcpyAsync(o, partial + 4 * (size_t)nb, sizeof o, hipMemcpyDeviceToHost, s));
+  AMG_HIP(hipStreamSynchronize(s));
+  if (avg) *avg = o[0] / o[1];
+  if (maxabs) *maxabs = o[2];
+  if (loc) {
+    const long long e = (long long)o[3];
+    loc[0] = (int)(e % nx) + 2;
+    loc[1] = (int)((e / nx) % ny) + 2;
+    loc[2] = (int)(e / ((long long)nx * ny)) + 2;
+  }
+  return AMG_OK;
+}
 
 extern "C" {
 
@@ -250,9 +281,13 @@ int amg_create(const amg_config* cfg, const amg_grid* grid, amg_handle* out) {
   if (cfg->precision != AMG_FP64 && cfg->precision != AMG_MIXED)
     return fail(AMG_EINVAL, "amg_create: unknown precision");
   if (cfg->maxcy < 1 || !(cfg->tol >= 0.0)) return fail(AMG_EINVAL, "amg_create: bad maxcy/tol");
+  // the grid is validated on the host before any device work
+  Hierarchy H;
+  int rc = check_grid(*cfg, *grid, H);
+  if (rc != AMG_OK) return rc;
   amg_solver_s* h = new amg_solver_s();
-  int rc = h->ctx.init(*cfg);
-  if (rc == AMG_OK) rc = h->ctx.add_instance(*grid, &h->I);
+  rc = h->ctx.init(*cfg);
+  if (rc == AMG_OK) rc = h->ctx.add_instance(*grid, &h->I, &H);
   if (rc != AMG_OK) {
     delete h;
     return rc;
@@ -266,7 +301,6 @@ int amg_destroy(amg_handle h) {
   if (h->d_phi_stage) (void)hipFree(h->d_phi_stage);
   if (h->d_rhs_stage) (void)hipFree(h->d_rhs_stage);
   if (h->d_div_partial) (void)hipFree(h->d_div_partial);
-  if (h->h_div) (void)hipHostFree(h->h_div);
   if (h->d_cn) (void)hipFree(h->d_cn);
   if (h->h_cn) (void)hipHostFree(h->h_cn);
   delete h;
@@ -330,20 +364,6 @@ int amg_solve_host(amg_handle h, double* phif, const double* rhsf, int* iter, do
 }
 
 // ---- projection ----------------------------------------------------------
-static void metric_ptrs(const Instance* I, const double** rho, const double** rdxc,
-                        const double** rdyc, const double** rdzc, const double** rdxe,
-                        const double** rdye, const double** rdze) {
-  const int nx = I->nx, ny = I->ny, nz = I->nz;
-  const double* m = I->d_met;
-  *rho = m; m += nz + 2;
-  *rdxc = m; m += nx + 2;
-  *rdyc = m; m += ny + 2;
-  *rdzc = m; m += nz + 2;
-  *rdxe = m; m += nx + 2;
-  *rdye = m; m += ny + 2;
-  *rdze = m;
-}
-
 int amg_psource_device(amg_handle h, const double* U, const double* V, const double* W, double* SRC,
                        double dt) {
   if (!h || !U || !V || !W || !SRC || dt == 0.0) return fail(AMG_EINVAL, "amg_psource_device: bad argument");
@@ -449,29 +469,10 @@ int amg_cn_advance_device(amg_handle h, const double* veltemp1, double* veltemp2
 int amg_divergence_device(amg_handle h, const double* U, const double* V, const double* W,
                           double* div, double* avg, double* maxabs, int loc[3]) {
   if (!h || !U || !V || !W) return fail(AMG_EINVAL, "amg_divergence_device: NULL argument");
-  const double *rho, *rdxc, *rdyc, *rdzc, *rdxe, *rdye, *rdze;
-  metric_ptrs(h->I, &rho, &rdxc, &rdyc, &rdzc, &rdxe, &rdye, &rdze);
-  const int nx = h->I->nx, ny = h->I->ny, nz = h->I->nz;
-  const int nb = divergence_blocks(nx, ny, nz);
-  if (!h->d_div_partial) {
+  const int nb = divergence_blocks(h->I->nx, h->I->ny, h->I->nz);
+  if (!h->d_div_partial)
     AMG_HIP(hipMalloc((void**)&h->d_div_partial, sizeof(double) * (4 * (size_t)nb + 4)));
-    AMG_HIP(hipHostMalloc((void**)&h->h_div, sizeof(double) * 4, hipHostMallocDefault));
-  }
-  hipStream_t s = h->ctx.stream;
-  launch_divergence(U, V, W, div, rdxc, rdyc, rdzc, nx, ny, nz, h->d_div_partial, s);
-  launch_divergence_finalize(h->d_div_partial, nb, h->d_div_partial + 4 * (size_t)nb, s);
-  AMG_HIP(hipMemcpyAsync(h->h_div, h->d_div_partial + 4 * (size_t)nb, 32, hipMemcpyDeviceToHost, s));
-  AMG_HIP(hipStreamSynchronize(s));
-  const double* o = h->h_div;
-  if (avg) *avg = o[0] / o[1];
-  if (maxabs) *maxabs = o[2];
-  if (loc) {
-    const long long e = (long long)o[3];
-    loc[0] = (int)(e % nx) + 2;
-    loc[1] = (int)((e / nx) % ny) + 2;
-    loc[2] = (int)(e / ((long long)nx * ny)) + 2;
-  }
-  return AMG_OK;
+  return divergence_report(h, U, V, W, div, h->d_div_partial, avg, maxabs, loc);
 }
 
 // ---- batch ---------------------------------------------------------------

@@ -161,8 +161,11 @@ constexpr int kZSeg = 16;
 constexpr int kZChunk32 = AMG_ZCHUNK32;
 constexpr int kZChunk64 = AMG_ZCHUNK64;
 constexpr int kZMaxSegs = 16;
+// form: 0 the launch's choice (FP32: the lean form when the launch has more
+// blocks than the device has CUs), 1 the short form (FP32 only), 2 the lean
+// form.  Both forms give the same bits.
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
-                  int zero_guess, int segs, hipStream_t s);
+                  int zero_guess, int segs, hipStream_t s, int form = 0);
 void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                      hipStream_t s);
 // Residual + restriction marching in z, for a transfer with factor 2 in every
@@ -309,6 +312,9 @@ void launch_psource(const double* U, const double* V, const double* W, double* S
 void launch_pgrad(const double* p, double* U, double* V, double* W, const double* rho,
                   const double* rdxe, const double* rdye, const double* rdze, int nx, int ny,
                   int nz, double dt, int wallmask, hipStream_t s);
+// kDivPer cells per thread: blocks = cdiv(cells, kBlock*kDivPer), each
+// writes 4 partials; the finalize writes 4 values.
+constexpr int kDivPer = 16;
 int divergence_blocks(int nx, int ny, int nz);
 void launch_divergence(const double* U, const double* V, const double* W, double* div,
                        const double* rdxc, const double* rdyc, const double* rdzc, int nx,

@@ -1803,6 +1803,16 @@ __global__ __launch_bounds__(kBlock) void k_pgrad(const double* __restrict__ p,
 }
 
 // Partials per block: {sum |div| vol, sum vol, max |div|, index of the max}.
+// Divergence report (dns/Routines.F90:123-173): a block covers
+// kDivPer*kBlock consecutive interior cells (x fastest), each thread
+// kDivPer of them at a stride of kBlock (coalesced rows), folded in a fixed
+// order.  Per block: {sum |div| V, sum V, max |div|, its cell index}; the
+// maximum keeps the smallest index on ties (the reference's first-found
+// order).
+__device__ __forceinline__ void div_better(double& b, long long& bi, double ob, long long oi) {
+  if (ob > b || (ob == b && oi < bi)) { b = ob; bi = oi; }
+}
+
 __global__ __launch_bounds__(kBlock) void k_divergence(const double* __restrict__ U,
                                                        const double* __restrict__ V,
                                                        const double* __restrict__ W,
@@ -1811,30 +1821,35 @@ __global__ __launch_bounds__(kBlock) void k_divergence(const double* __restrict_
                                                        const double* __restrict__ rdyc,
                                                        const double* __restrict__ rdzc, int nx,
                                                        int ny, int nz, double* __restrict__ partial) {
-  const int e0 = blockIdx.x * kBlock + threadIdx.x;
+  const long long n = (long long)nx * ny * nz;
+  const long long base = (long long)blockIdx.x * kBlock * kDivPer + threadIdx.x;
+  const int sx = nx + 2, sy = ny + 2;
+  const size_t sxy = (size_t)sx * sy;
   double s[2] = {0.0, 0.0};
   double best = -1.0;
   long long bidx = (long long)1 << 62;
-  if (e0 < nx * ny * nz) {
-    const int i = e0 % nx + 1, jk = e0 / nx, j = jk % ny + 1, k = jk / ny + 1;
-    const int sx = nx + 2, sy = ny + 2;
-    const size_t sxy = (size_t)sx * sy;
-    const size_t e = (size_t)k * sxy + (size_t)j * sx + i;
-    const double d = (U[e] - U[e - 1]) * rdxc[i] + (V[e] - V[e - sx]) * rdyc[j] +
-                     (W[e] - W[e - sxy]) * rdzc[k];
-    if (div) div[e] = d;
-    const double vinv = rdxc[i] * rdyc[j] * rdzc[k];
-    s[0] = fabs(d) / vinv;
-    s[1] = 1.0 / vinv;
-    best = fabs(d);
-    bidx = e0;
+#pragma unroll 4
+  for (int q = 0; q < kDivPer; ++q) {
+    const long long e0 = base + (long long)q * kBlock;
+    if (e0 < n) {
+      const int i = (int)(e0 % nx) + 1;
+      const long long jk = e0 / nx;
+      const int j = (int)(jk % ny) + 1, k = (int)(jk / ny) + 1;
+      const size_t e = (size_t)k * sxy + (size_t)j * sx + i;
+      const double d = (U[e] - U[e - 1]) * rdxc[i] + (V[e] - V[e - sx]) * rdyc[j] +
+                       (W[e] - W[e - sxy]) * rdzc[k];
+      if (div) div[e] = d;
+      const double vinv = rdxc[i] * rdyc[j] * rdzc[k];
+      s[0] += fabs(d) / vinv;
+      s[1] += 1.0 / vinv;
+      div_better(best, bidx, fabs(d), e0);
+    }
   }
-  // max with the smallest index on ties (the reference's first-found order)
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const double ob = __shfl_xor(best, off, 64);
     const long long oi = __shfl_xor(bidx, off, 64);
-    if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+    div_better(best, bidx, ob, oi);
   }
   __shared__ double sb[kBlock / 64];
   __shared__ long long si[kBlock / 64];
@@ -1845,24 +1860,48 @@ __global__ __launch_bounds__(kBlock) void k_divergence(const double* __restrict_
   if (threadIdx.x == 0) {
     double b = sb[0];
     long long bi = si[0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
-      if (sb[q] > b || (sb[q] == b && si[q] < bi)) { b = sb[q]; bi = si[q]; }
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) div_better(b, bi, sb[q], si[q]);
     out[2] = b;
     out[3] = (double)bi;
   }
 }
 
-__global__ void k_divergence_finalize(const double* __restrict__ partial, int nb,
-                                      double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
-  double s0 = 0.0, s1 = 0.0, b = -1.0, bi = 0.0;
-  for (int q = 0; q < nb; ++q) {
+// One block of kDivFin threads: thread t folds partials t, t+kDivFin, ...
+// in order, then a fixed-shape tree combines the threads (deterministic).
+constexpr int kDivFin = 1024;
+__global__ __launch_bounds__(kDivFin) void k_divergence_finalize(const double* __restrict__ partial,
+                                                                 int nb, double* __restrict__ out) {
+  __shared__ double s0s[kDivFin / 64], s1s[kDivFin / 64], bs[kDivFin / 64];
+  __shared__ long long is[kDivFin / 64];
+  double s0 = 0.0, s1 = 0.0, b = -1.0;
+  long long bi = (long long)1 << 62;
+  for (int q = threadIdx.x; q < nb; q += kDivFin) {
     const double* p = partial + (size_t)q * 4;
     s0 += p[0];
     s1 += p[1];
-    if (p[2] > b || (p[2] == b && p[3] < bi)) { b = p[2]; bi = p[3]; }
+    div_better(b, bi, p[2], (long long)p[3]);
   }
-  out[0] = s0; out[1] = s1; out[2] = b; out[3] = bi;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    const double ob = __shfl_xor(b, off, 64);
+    const long long oi = __shfl_xor(bi, off, 64);
+    div_better(b, bi, ob, oi);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { s0s[w] = s0; s1s[w] = s1; bs[w] = b; is[w] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a0 = s0s[0], a1 = s1s[0], ab = bs[0];
+    long long ai = is[0];
+    for (int q = 1; q < kDivFin / 64; ++q) {
+      a0 += s0s[q];
+      a1 += s1s[q];
+      div_better(ab, ai, bs[q], is[q]);
+    }
+    out[0] = a0; out[1] = a1; out[2] = ab; out[3] = (double)ai;
+  }
 }
 
 inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
@@ -1898,7 +1937,7 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
   else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
 }
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
-                  int segs, hipStream_t s) {
+                  int segs, hipStream_t s, int form) {
   const dim3 blk(64, segs);
   // reduced-system state (two buffers in FP32, one in FP64) + staged rows
   const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
@@ -1913,7 +1952,7 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
   if (fp64) {
     if (nbr) ZL(double, kZChunk64, true, true);
     else ZL(double, kZChunk64, false, true);
-  } else if (ll.nblocks > device_cus()) {
+  } else if (form == 2 || (form == 0 && ll.nblocks > device_cus())) {
     if (nbr) ZL(float, kZChunk32, true, true);
     else ZL(float, kZChunk32, false, true);
   } else {
@@ -2091,7 +2130,9 @@ void launch_pgrad(const double* p, double* U, double* V, double* W, const double
   LAUNCH(k_pgrad, nb((size_t)nx * ny * nz), kBlock, p, U, V, W, rho, rdxe, rdye, rdze, nx, ny, nz, dt,
          wallmask);
 }
-int divergence_blocks(int nx, int ny, int nz) { return (int)nb((size_t)nx * ny * nz); }
+int divergence_blocks(int nx, int ny, int nz) {
+  return (int)nb((size_t)nx * ny * nz, kBlock * kDivPer);
+}
 void launch_divergence(const double* U, const double* V, const double* W, double* div,
                        const double* rdxc, const double* rdyc, const double* rdzc, int nx, int ny,
                        int nz, double* partial, hipStream_t s) {
@@ -2099,7 +2140,7 @@ void launch_divergence(const double* U, const double* V, const double* W, double
          nz, partial);
 }
 void launch_divergence_finalize(const double* partial, int nblocks, double* out, hipStream_t s) {
-  LAUNCH(k_divergence_finalize, 1, 64, partial, nblocks, out);
+  LAUNCH(k_divergence_finalize, 1, kDivFin, partial, nblocks, out);
 }
 
 }  // namespace amg

@@ -86,6 +86,14 @@ int amg_server_s::bind(Member& m) {
   const double ncell = (double)m.grid.nx * m.grid.ny * m.grid.nz;
   const double w = ctx.cfg.precision == AMG_FP64 ? 8.0 : 4.0;
   const double est = ncell * (8.0 / 7.0) * w * 2.5 + ncell * 8.0 * (ctx.cfg.precision == AMG_FP64 ? 1.0 : 3.0);
+  // warm-start pressures stay resident; a hierarchy that does not fit beside
+  // them even with every other hierarchy evicted never will: the request fails
+  double warm = 0.0;
+  for (auto& x : members) warm += (double)x.nghost * 8.0;
+  if (warm + est > budget) {
+    set_error("amg_server: the member's hierarchy does not fit the HBM budget");
+    return AMG_ENOMEM;
+  }
   while (bound() >= max_slots || (double)used + est > budget) {
     Member* victim = nullptr;
     for (auto& x : members)

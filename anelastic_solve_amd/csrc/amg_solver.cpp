@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 
 namespace amg {
 
@@ -68,6 +69,16 @@ void DevArena::release() {
   bytes = 0;
 }
 
+size_t partial_blocks(const Instance& I) {
+  const HostLevel& F = I.H.lv[0];
+  const long long n = F.ncell();
+  long long b = march_blocks(F.nx / 2, F.ny, F.nz);
+  b = std::max(b, (long long)cdiv(n, (long long)kBlock * kFinePer));
+  b = std::max(b, (long long)cdiv(n, (long long)kBlock * kWsumPer));
+  b = std::max(b, (long long)cdiv(n, kBlock));
+  return (size_t)b + 1;
+}
+
 size_t cycle_bytes(const Instance& I) {
   const double w = I.fp64 ? 8.0 : 4.0;
   double b = 0.0;
@@ -81,11 +92,15 @@ size_t cycle_bytes(const Instance& I) {
 
 int Context::init(const amg_config& c) {
   cfg = c;
+  // the relmax stop rule needs the solution change, which an FP64 solve
+  // computes only when it reports relmax
+  if (cfg.stop_rule == AMG_STOP_RELMAX && cfg.precision == AMG_FP64) cfg.report_relmax = 1;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
     set_error("no HIP device visible");
     return AMG_ENOGPU;
   }
+  AMG_HIP(hipGetDevice(&device));
   AMG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   AMG_HIP(hipHostMalloc((void**)&h_out, sizeof(double) * kOutRegion * 4, hipHostMallocDefault));
   AMG_HIP(hipMalloc((void**)&d_out, sizeof(double) * kOutRegion * 4));
@@ -121,25 +136,95 @@ Context::~Context() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+// Removes instance I and compacts the descriptor tables to the live
+// instances' entries (re-indexed in instance order), so that a server that
+// binds and evicts hierarchies for ever keeps tables of bounded size.  The
+// device tables are re-uploaded; no work of the context may be in flight.
 void Context::remove_instance(Instance* I) {
   auto it = std::find(inst.begin(), inst.end(), I);
   if (it != inst.end()) inst.erase(it);
   delete I;
+  std::vector<DevLevel> nL;
+  std::vector<DevXfer> nX;
+  std::vector<DevTail> nT;
+  for (Instance* J : inst) {
+    std::vector<int> olds(J->lev.begin(), J->lev.end());
+    if (!J->fp64) {
+      olds.push_back(J->fine[0]);
+      olds.push_back(J->fine[1]);
+    }
+    auto remap = [&](int o) {
+      for (size_t q = 0; q < olds.size(); ++q)
+        if (olds[q] == o) return (int)nL.size() + (int)q;
+      return -1;
+    };
+    const int xb = (int)nX.size();
+    std::vector<int> oldx(J->xf.begin(), J->xf.end());
+    auto remapx = [&](int o) {
+      for (size_t q = 0; q < oldx.size(); ++q)
+        if (oldx[q] == o) return xb + (int)q;
+      return -1;
+    };
+    std::vector<DevLevel> mine;
+    for (int o : olds) {
+      DevLevel d = hL[o];
+      if (d.link >= 0) d.link = remap(d.link);
+      mine.push_back(d);
+    }
+    for (int o : oldx) {
+      DevXfer x = hX[o];
+      x.fine = remap(x.fine);
+      x.coarse = remap(x.coarse);
+      nX.push_back(x);
+    }
+    DevTail t = hT[J->tail];
+    for (int d = 0; d < t.nlev; ++d) {
+      t.lev[d] = remap(t.lev[d]);
+      if (d + 1 < t.nlev) t.xf[d] = remapx(t.xf[d]);
+    }
+    for (int& l : J->lev) l = remap(l);
+    J->fine[0] = remap(J->fine[0]);
+    J->fine[1] = remap(J->fine[1]);
+    for (int& x : J->xf) x = remapx(x);
+    J->tail = (int)nT.size();
+    nT.push_back(t);
+    nL.insert(nL.end(), mine.begin(), mine.end());
+  }
+  hL.swap(nL);
+  hX.swap(nX);
+  hT.swap(nT);
+  if (!inst.empty()) (void)upload_tables();
+  else drop_graphs();
 }
 
-int Context::add_instance(const amg_grid& g, Instance** out) {
-  Instance* I = new Instance();
+// The new instance's descriptors are collected in local tables and appended
+// to hL/hX/hT only once everything is allocated and uploaded: an error on
+// the way frees the instance (and its device memory) and leaves the
+// context's tables as they were.
+int check_grid(const amg_config& cfg, const amg_grid& g, Hierarchy& H) {
   std::string err;
-  int rc = build_hierarchy(cfg, g, I->H, err);
+  const int rc = build_hierarchy(cfg, g, H, err);
   if (rc != AMG_OK) {
     set_error(err);
-    delete I;
     return rc;
   }
   if (cfg.smoother == AMG_SMOOTHER_LJ && g.nz > kZSeg * kZMaxSegs) {
-    set_error("amg_create: the z-line smoother supports nz <= 256");
-    delete I;
+    set_error("amg_create: the z-line smoother (LJ) supports nz <= " +
+              std::to_string(kZSeg * kZMaxSegs) + " (got nz=" + std::to_string(g.nz) +
+              "); use RBPGS or split the column");
     return AMG_EBADSIZE;
+  }
+  return AMG_OK;
+}
+
+int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
+  std::unique_ptr<Instance> hold(new Instance());
+  Instance* I = hold.get();
+  if (built) {
+    I->H = std::move(*built);
+  } else {
+    const int rc = check_grid(cfg, g, I->H);
+    if (rc != AMG_OK) return rc;
   }
   I->id = inst.empty() ? 0 : inst.back()->id + 1;
   I->nx = g.nx; I->ny = g.ny; I->nz = g.nz;
@@ -149,7 +234,6 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
   const size_t w = I->fp64 ? 8 : 4;
   auto fail = [&]() {
     set_error("device allocation failed");
-    delete I;
     return AMG_ENOMEM;
   };
   // coefficient tables, volumes, density, folds
@@ -236,28 +320,33 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     fA.u = pA; fA.u2 = pB; fA.f = b; fA.scr = nullptr;
     fB.u = pB; fB.u2 = pA; fB.f = b; fB.scr = nullptr;
   }
-  const int base = (int)hL.size();
+  // descriptor indices: this instance's entries follow the context's
+  const int base = (int)hL.size(), xbase = (int)hX.size();
+  std::vector<DevLevel> newL;
+  std::vector<DevXfer> newX;
   for (int l = 0; l < L; ++l) {
     I->lev.push_back(base + l);
-    hL.push_back(ents[l]);
+    newL.push_back(ents[l]);
   }
   if (I->fp64) {
     I->fine[0] = I->fine[1] = I->lev[0];
   } else {
     fA.link = fB.link = I->lev[0];
-    I->fine[0] = (int)hL.size();
-    hL.push_back(fA);
-    I->fine[1] = (int)hL.size();
-    hL.push_back(fB);
+    I->fine[0] = base + (int)newL.size();
+    newL.push_back(fA);
+    I->fine[1] = base + (int)newL.size();
+    newL.push_back(fB);
   }
   for (int l = 0; l + 1 < L; ++l) {
     DevXfer x;
     x.fine = I->lev[l];
     x.coarse = I->lev[l + 1];
     x.fx = I->H.lv[l].fx; x.fy = I->H.lv[l].fy; x.fz = I->H.lv[l].fz;
-    I->xf.push_back((int)hX.size());
-    hX.push_back(x);
+    I->xf.push_back(xbase + (int)newX.size());
+    newX.push_back(x);
   }
+  DevTail tl;
+  std::memset(&tl, 0, sizeof tl);
   // coarse tail: every level from the first one with <= kTailCells cells
   {
     // grow the tail upward from the coarsest level while it stays small,
@@ -285,12 +374,9 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     }
     if (bytes > limit) {
       set_error("amg_create: the coarsest level does not fit in LDS");
-      delete I;
       return AMG_EBADSIZE;
     }
     I->l0 = l0;
-    DevTail tl;
-    std::memset(&tl, 0, sizeof tl);
     tl.nlev = L - l0;
     const bool zero_first = l0 > 0 || !I->fp64;   // the first tail level starts from u = 0
     tl.copy_u_in = zero_first ? 0 : 1;
@@ -341,7 +427,6 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
       }
       if ((int)st.size() > kTailMaxSteps) {
         set_error("amg_create: too many smoothing sweeps on the coarse levels");
-        delete I;
         return AMG_EINVAL;
       }
       tl.nsteps = (int)st.size();
@@ -359,7 +444,6 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     I->tail_split = tl.lds_minv < 0 && (int)nco >= kSplitCells;
     I->tail = (int)hT.size();
     I->tail_lds = off;
-    hT.push_back(tl);
   }
   // projection metrics
   {
@@ -378,9 +462,17 @@ int Context::add_instance(const amg_grid& g, Instance** out) {
     AMG_HIP(hipMemcpy(I->d_met, m.data(), m.size() * 8, hipMemcpyHostToDevice));
   }
   I->bytes_per_cycle = cycle_bytes(*I);
-  inst.push_back(I);
-  int rc2 = upload_tables();
-  if (rc2 != AMG_OK) return rc2;
+  hL.insert(hL.end(), newL.begin(), newL.end());
+  hX.insert(hX.end(), newX.begin(), newX.end());
+  hT.push_back(tl);
+  inst.push_back(hold.release());
+  const int rc2 = upload_tables();
+  if (rc2 != AMG_OK) {
+    const std::string e = last_error();
+    remove_instance(I);
+    set_error(e);
+    return rc2;
+  }
   *out = I;
   return AMG_OK;
 }
@@ -405,9 +497,11 @@ int Context::upload_tables() {
   AMG_HIP(hipMemcpy(dL, hL.data(), hL.size() * sizeof(DevLevel), hipMemcpyHostToDevice));
   if (!hX.empty())
     AMG_HIP(hipMemcpy(dX, hX.data(), hX.size() * sizeof(DevXfer), hipMemcpyHostToDevice));
-  // partial buffer: enough for every instance's finest level at 4 values per block
+  // partial buffer: 4 values per block of every instance's largest
+  // finest-level launch (the z-marching kernels, the change and the
+  // weighted-sum folds)
   size_t need = 0;
-  for (Instance* I : inst) need += (size_t)cdiv(I->H.lv[0].ncell(), kBlock) + 1;
+  for (Instance* I : inst) need += partial_blocks(*I);
   need *= 4;
   if (need > cap_partial) {
     if (d_partial) AMG_HIP(hipFree(d_partial));
@@ -419,7 +513,8 @@ int Context::upload_tables() {
 }
 
 // z-line launches: items grouped by their segment count (blockDim.y).
-void Context::zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg) {
+void Context::zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg,
+                             int form) {
   std::vector<Item> bucket[kZMaxSegs + 1];
   for (const Item& it : items) {
     const int nz = hL[it.idx].nz;
@@ -429,7 +524,7 @@ void Context::zline_launches(const std::vector<Item>& items, bool fp64, int pari
   for (int sg = 1; sg <= kZMaxSegs; ++sg)
     if (!bucket[sg].empty())
       for_lists(bucket[sg], [&](const LaunchList& ll, int, int) {
-        launch_zline(fp64, dL, ll, parity, zg, sg, stream);
+        launch_zline(fp64, dL, ll, parity, zg, sg, stream, form);
       });
 }
 
@@ -936,6 +1031,16 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
   const bool zline = cfg.smoother == AMG_SMOOTHER_LJ;
   std::vector<Item> items;
   double b = 0.0;
+  int form = 0;
+  if (which >= 10 && which <= 13) {
+    // the z-line half-sweep in a given form: 10/11 short, 12/13 lean (red/black)
+    if (!zline || (fp64 && which < 12)) {
+      set_error("amg_run_kernel: forms 10-13 are z-line sweeps; the short form is FP32 only");
+      return AMG_EINVAL;
+    }
+    form = which < 12 ? 1 : 2;
+    which = (which & 1) ? 4 : 0;
+  }
   if (which == 0 || which == 4 || which == 6) {
     items.push_back({I->lev[level], zline ? cdiv((long long)d.hx * d.ny, 64)
                                           : cdiv((long long)d.hx * d.ny * d.nz, kBlock)});
@@ -977,7 +1082,7 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     for_lists(items, [&](const LaunchList& ll, int, int) {
       if (which == 0 || which == 4 || which == 6) {
         const int par = which == 4 ? 1 : 0, zg = which == 6 ? 3 : 0;
-        if (zline) zline_launches(items, fp64, par, zg);
+        if (zline) zline_launches(items, fp64, par, zg, form);
         else launch_rb_point(fp64, dL, ll, par, zg, stream);
       } else if (which == 7 || which == 8) {
         launch_zline_col(fp64, dL, ll, which == 8 ? 1 : 0, 0, stream);

@@ -77,6 +77,7 @@ struct Instance {
 // A set of instances sharing one stream and one descriptor table.
 struct Context {
   amg_config cfg{};
+  int device = 0;                // the HIP device current at init (the stream's)
   hipStream_t stream = nullptr;
   std::vector<Instance*> inst;
   std::vector<DevLevel> hL;
@@ -103,12 +104,13 @@ struct Context {
 
   int init(const amg_config& c);
   ~Context();
-  int add_instance(const amg_grid& g, Instance** out);   // builds + allocates
+  // builds the hierarchy (or takes `built`, from check_grid) and allocates
+  int add_instance(const amg_grid& g, Instance** out, Hierarchy* built = nullptr);
   int upload_tables();
   int solve(const std::vector<Instance*>& S, amg_stats* st);
   // pieces
   void vcycle(const std::vector<Instance*>& S, bool fp64);
-  void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
+  void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg, int form = 0);
   int prepare(const std::vector<Instance*>& S);
   const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
@@ -120,6 +122,14 @@ struct Context {
   void remove_instance(Instance* I);
 };
 
+// Host-side validation of a grid under cfg (the checks of
+// mgMK/mg_check_size.F90:43-151 and this build's own limits), building its
+// hierarchy; no device work.  AMG_OK or an error code with the message set.
+int check_grid(const amg_config& cfg, const amg_grid& g, Hierarchy& H);
+
 size_t cycle_bytes(const Instance& I);
+// Blocks of the instance's largest finest-level launch that writes block
+// partials (each writes at most 4 values per block).
+size_t partial_blocks(const Instance& I);
 
 }  // namespace amg

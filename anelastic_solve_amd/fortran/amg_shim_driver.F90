@@ -12,6 +12,14 @@
 !       directions, p = 1 and p = 2 at the ghost-cell centres of the two
 !       Dirichlet faces, so p = z^2 + b1 z + c1 exactly for the 3-point
 !       stencil.  Prints PASS/FAIL with the max error; exit status 0 on PASS.
+!   amg_shim_driver relmax <fp64|mixed> <work.ini>
+!       the same case (x3) with the reference's own stop rule only: the INI
+!       has tolmax and no tolres, so mg_solver stops on the relative
+!       solution change (mgMK/mg_solver.F90:127).  PASS when it converges
+!       before maxcy with 0 < relmax <= tolmax and the closed form holds.
+!   amg_shim_driver rho3d <work.ini>
+!       MG_init with a background density r that varies horizontally:
+!       prints the status (AMG_EUNSUPPORTED); needs no GPU.
 program amg_shim_driver
   use iso_c_binding
   use amg_cabi
@@ -35,9 +43,14 @@ program amg_shim_driver
      call mg_read_params(trim(a2), stat)
      call print_params(stat)
   case ('dirichlet')
-     call dirichlet(trim(a2), trim(a3), trim(a4))
+     call dirichlet(trim(a2), trim(a3), trim(a4), .false.)
+  case ('relmax')
+     call dirichlet('x3', trim(a2), trim(a3), .true.)
+  case ('rho3d')
+     call rho3d(trim(a2))
   case default
      write(0, '(a)') 'usage: amg_shim_driver layout | params <ini> | dirichlet <x1|x3> <fp64|mixed> <ini>'
+     write(0, '(a)') '     | relmax <fp64|mixed> <ini> | rho3d <ini>'
      error stop 2
   end select
 
@@ -96,8 +109,51 @@ contains
     de = h; dc = h; rde = 1.0_r8 / h; rdc = 1.0_r8 / h
   end subroutine uniform
 
-  subroutine dirichlet(dir, prec, ini)
+  subroutine rho3d(ini)
+    character(len=*), intent(in) :: ini
+    integer, parameter :: n = 16
+    integer :: i, st, u
+    nx = n; ny = n; nz = n
+    nxp2 = n + 2; nyp2 = n + 2; nzp2 = n + 2
+    sx = 2; ex = n + 1; sy = 2; ey = n + 1; sz = 2; ez = n + 1
+    xL = 1.0_r8; yL = 1.0_r8; zL = 1.0_r8
+    call uniform(n, xe, xc, dxe, dxc, rdxe, rdxc)
+    call uniform(n, ye, yc, dye, dyc, rdye, rdyc)
+    call uniform(n, ze, zc, dze, dzc, rdze, rdzc)
+    allocate(r(sx-1:ex+1, sy-1:ey+1, sz-1:ez+1))
+    do i = sx - 1, ex + 1
+       r(i, :, :) = 1.0_r8 + 0.01_r8 * xc(i)
+    end do
+    open(newunit=u, file=ini, status='replace', action='write')
+    write(u, '(a)') '[MGsolver]'
+    write(u, '(a)') 'Smoother=LJ'
+    write(u, '(a)') 'nxlevels=3'
+    write(u, '(a)') 'nylevels=3'
+    write(u, '(a)') 'nzlevels=3'
+    write(u, '(a)') 'maxcy=20'
+    write(u, '(a)') 'tolmax=1.0d-6'
+    write(u, '(a)') 'PreRelax=2,2,2'
+    write(u, '(a)') 'PstRelax=2,2,2'
+    write(u, '(a)') '[BCs]'
+    write(u, '(a)') 'X1=periodic'
+    write(u, '(a)') 'X2=periodic'
+    write(u, '(a)') 'X3=read'
+    write(u, '(a)') '[X3min]'
+    write(u, '(a)') 'PTYPE=Neumann'
+    write(u, '(a)') 'PVALUE=0.d0'
+    write(u, '(a)') '[X3max]'
+    write(u, '(a)') 'PTYPE=Neumann'
+    write(u, '(a)') 'PVALUE=0.d0'
+    close(u)
+    call mg_read_params(ini, st)
+    if (st /= 0) error stop 3
+    call MG_init(st)
+    print '(a,1x,i0)', 'STAT', st
+  end subroutine rho3d
+
+  subroutine dirichlet(dir, prec, ini, relmax_only)
     character(len=*), intent(in) :: dir, prec, ini
+    logical, intent(in) :: relmax_only
     integer, parameter :: n = 32
     real(r8), parameter :: a1 = 2.0_r8, va = 1.0_r8, vb = 2.0_r8
     real(r8), allocatable, target :: phi(:,:,:), src(:,:,:)
@@ -126,8 +182,12 @@ contains
     write(u, '(a)') 'nylevels=4'
     write(u, '(a)') 'nzlevels=4'
     write(u, '(a)') 'maxcy=60'
-    write(u, '(a)') 'tolmax=1.0d-13'
-    write(u, '(a)') 'tolres=1.0d-12'
+    if (relmax_only) then
+       write(u, '(a)') 'tolmax=1.0d-12'
+    else
+       write(u, '(a)') 'tolmax=1.0d-13'
+       write(u, '(a)') 'tolres=1.0d-12'
+    end if
     write(u, '(a)') 'Precision='//prec
     write(u, '(a)') 'PreRelax=2,2,2,2'
     write(u, '(a)') 'PstRelax=2,2,2,2'
@@ -200,6 +260,10 @@ contains
        err = max(err, maxval(abs(phi(sx-1, sy:ey, sz:ez) - phi(ex, sy:ey, sz:ez))))
     end if
     call mg_end(st)
+    if (relmax_only .and. .not. (relmax > 0.0_r8 .and. relmax <= 1.0e-12_r8 .and. iter < 60)) then
+       print '(a,1x,a,1x,a,1x,i0,2es12.4)', 'FAIL', dir, prec, iter, err / pmax, relmax
+       error stop 1
+    end if
     if (err / pmax <= 1.0e-9_r8 .and. iter >= 1 .and. iter <= 60 .and. st == 0) then
        print '(a,1x,a,1x,a,1x,i0,2es12.4)', 'PASS', dir, prec, iter, err / pmax, relmax
     else

@@ -4,9 +4,13 @@
 Workload (the headline configuration): 256^3 cells on 2*pi x 2*pi x 1,
 periodic x/y, Neumann z, tanh vertical stretching (beta = 2, clustered at
 the mid-plane), RHS = divergence of a seeded random velocity with a k^-5/3
-spectrum in x/y.  A step is one solve from a zero initial guess to a
-relative residual ||b - A p||_inf / ||b||_inf <= 1e-10 with V(2,2) cycles
-and the z-line red-black Gauss-Seidel smoother, inputs resident in HBM.
+spectrum in x/y.  A step is one solve from a zero initial guess with
+V(2,2) cycles and the z-line red-black Gauss-Seidel smoother, inputs
+resident in HBM, to a relative residual ||b - A p||_inf / ||b||_inf of
+1e-12: at the workload's 1e-10 the pressure is still 9e-9 (relative,
+max-norm) from the exact answer on this stretched grid, and 1e-12 is the
+stop at which it is within the 1e-9 bar.  The 1e-10 rates are reported
+beside it (at_workload_tol).
 
 ``value`` is solves/s of the FP32-smoothed / FP64-corrected solver; the
 plain FP64 V-cycle runs on the same input for the comparison.  The JSON
@@ -37,7 +41,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, nargs=3, default=[256, 256, 256])
     ap.add_argument("--beta", type=float, default=2.0)
-    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--tol", type=float, default=None,
+                    help="the parity stop: the relative residual at which the pressure is "
+                         "within 1e-9 of the tight FP64 solve (value is measured here)")
+    ap.add_argument("--tol-workload", type=float, default=1e-10,
+                    help="the workload's stated stop, timed as a secondary rate")
     ap.add_argument("--smoother", default="LJ")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -324,10 +332,10 @@ def bench_production(torch, steps=5):
     return out
 
 
-def cpu_baseline(n, beta, tol, cycles, iters_needed):
+def cpu_baseline_port(n, beta, tol, cycles, iters_needed):
     """Bounded sample: `cycles` V-cycles of the numpy oracle (mixed precision,
     same grid and RHS) timed on the host; solves/s extrapolated with the GPU's
-    cycle count."""
+    cycle count.  Used only when the reference's own solver is not built."""
     import numpy as np
     from anelastic_solve_amd import box_grid, workloads
     from oracle import grid as G, ops, vcycle as VC
@@ -357,6 +365,95 @@ def cpu_baseline(n, beta, tol, cycles, iters_needed):
                        f"{n[0]}x{n[1]}x{n[2]}; {dt:.2f} s/cycle x {iters_needed} cycles per solve")}
 
 
+MGMK = os.path.join(ROOT, "oracle", "_ref", "mgmk_harness")
+MPIEXEC = "/opt/conda/bin/mpiexec"
+MGMK_CYCLES = os.path.join(ROOT, "tests", "golden", "mgmk_cycles.json")
+
+
+def _mgmk_ini(path, levels, maxcy):
+    lv = ",".join(["2"] * levels)
+    with open(path, "w") as fh:
+        fh.write("[MGsolver]\nSmoother=RBPGS\nStrict=yes\nVerbose=no\n"
+                 f"nxlevels={levels}\nnylevels={levels}\nnzlevels={levels}\n"
+                 f"maxcy={maxcy}\ntolmax=1.0d-14\nPreRelax={lv}\nPstRelax={lv}\n"
+                 "[BCs]\nX1=periodic\nX2=periodic\nX3=read\n"
+                 "[X3min]\nPTYPE=Neumann\nPVALUE=0.d0\n[X3max]\nPTYPE=Neumann\nPVALUE=0.d0\n")
+
+
+def _mgmk_time(n, levels, procs, maxcy, count, tmp, rhsfile):
+    """Best of `count` timed mg_solver calls of `maxcy` V-cycles (MPI_WTIME
+    around the call, as mgMK/test/test.F90:443-457), and the final relres."""
+    import subprocess
+    ini = os.path.join(tmp, f"mg{n}_{maxcy}.ini")
+    _mgmk_ini(ini, levels, maxcy)
+    np_ = procs[0] * procs[1] * procs[2]
+    cmd = [MPIEXEC, "-n", str(np_), MGMK, ini, str(n), str(n), str(n), *map(str, procs),
+           repr(2 * math.pi), repr(2 * math.pi), "1.0", rhsfile, "-", "time", str(count), tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("mgmk_harness failed: " + r.stderr[-500:])
+    row = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("TIMED")][0]
+    return float(row[3]), float(row[5])
+
+
+def cpu_baseline_reference(n=256, levels=7):
+    """The reference's own CPU solver (mgMK built from its Fortran sources,
+    oracle/_ref/mgmk_harness) on this host's cores: the uniform n^3
+    analogue of the workload (2*pi x 2*pi x 1, periodic x/y, Neumann z;
+    mgMK's operator is the uniform one, so it cannot take the stretched
+    grid), RHS = Psource of N(0,1) face velocities, RBPGS V(2,2), MPI ranks
+    split over x and y.  Bounded sample: mg_solver calls of 2 and 8
+    V-cycles, timed with MPI_WTIME around the call; a solve to a relative
+    residual of 1e-10 costs the 2-cycle call plus the marginal cycle time
+    for the rest of the cycles the reference needs (tests/golden/
+    mgmk_cycles.json, measured with the same build)."""
+    import tempfile
+    import numpy as np
+    from anelastic_solve_amd import workloads
+    from oracle import grid as G, ops
+    with open(MGMK_CYCLES) as fh:
+        need = json.load(fh)["sizes"][str(n)]["cycles_to_1e-10"]
+    cores = min(16, os.cpu_count() or 1)
+    px = 4 if cores >= 4 else cores
+    py = max(1, min(4, cores // px))
+    with tempfile.TemporaryDirectory() as tmp:
+        g, U, V, W = workloads.make(workloads.UNIFORM_64, n=(n, n, n))
+        b = workloads.psource_host(U, V, W, g)
+        axes = [G.axis_from_faces(a.xe[:n + 1]) for a in (g.ax, g.ay, g.az)]
+        levs = G.hierarchy(*axes, np.ones(n + 2), [0, 1, 0, 1, 0, 0], [])
+        b[1:-1, 1:-1, 1:-1] = ops.compat_project(b[1:-1, 1:-1, 1:-1], levs[0])
+        b[0, :, :] = b[-1, :, :] = 0.0
+        b[:, 0, :] = b[:, -1, :] = 0.0
+        b[:, :, 0] = b[:, :, -1] = 0.0
+        rf = os.path.join(tmp, "rhs.bin")
+        b.astype("<f8").tofile(rf)
+        del U, V, W, g, levs
+        k1, k2 = 2, 8
+        t1, _ = _mgmk_time(n, levels, (px, py, 1), k1, 2, tmp, rf)
+        t2, rr = _mgmk_time(n, levels, (px, py, 1), k2, 2, tmp, rf)
+    per = (t2 - t1) / (k2 - k1)
+    solve = t1 + (need - k1) * per
+    return {"value": 1.0 / solve, "unit": "solves/s", "cores": px * py, "kind": "reference",
+            "sample": (f"reference mgMK mg_solver (oracle/_ref, flang -O3, MPICH) on {px}x{py}x1 "
+                       f"ranks, uniform {n}^3 analogue (periodic x/y, Neumann z, RBPGS V(2,2)): "
+                       f"{k1}-cycle call {t1:.2f} s, {per:.3f} s per further cycle; it needs "
+                       f"{need} cycles to relres 1e-10 (tests/golden/mgmk_cycles.json), "
+                       f"so {solve:.1f} s per solve"),
+            "s_per_cycle": per, "cycles_to_1e-10": need, "relres_after_sample": rr,
+            "where": "this host, same run as the GPU measurement"}
+
+
+def cpu_baseline(n, beta, tol, cycles, iters_needed):
+    """The reference's own solver when it is built, else the numpy port."""
+    if os.path.exists(MGMK) and os.path.exists(MPIEXEC) and os.path.exists(MGMK_CYCLES) \
+            and tuple(n) == (256, 256, 256):
+        try:
+            return cpu_baseline_reference()
+        except Exception as e:          # noqa: BLE001 -- report, then fall back
+            print(f"# reference CPU baseline failed: {e}", file=sys.stderr, flush=True)
+    return cpu_baseline_port(n, beta, tol, cycles, iters_needed)
+
+
 def copy_calibration():
     import torch
     n = 64 * 1024 * 1024
@@ -381,16 +478,26 @@ def main():
     import torch
     from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
 
+    if a.tol is None:
+        a.tol = workloads.PARITY_TOL
     n = tuple(a.n)
     g = box_grid(*n, beta=a.beta)
     U, V, W = workloads.velocity_spectral(g, 1)
     rhs = torch.from_numpy(workloads.psource_host(U, V, W, g)).cuda()
     print(f"# workload {n} beta={a.beta} ready", file=sys.stderr, flush=True)
 
-    res = {}
-    sol = {}
-    for prec in (["mixed"] if a.no_fp64 else ["mixed", "fp64"]):
-        cfg = default_config(precision=prec, smoother=a.smoother, tol=a.tol, maxcy=200)
+    # the FP64 reference answer: a tight FP64 solve (null space removed)
+    cfg = default_config(precision="fp64", smoother=a.smoother, tol=1e-13, maxcy=200)
+    s = PressureSolver(g, cfg)
+    pt = torch.zeros_like(rhs)
+    s.mg_solver(pt, rhs)
+    tight_relres = s.stats.relres
+    s.close()
+    inner = (slice(1, -1),) * 3
+    pt_max = pt[inner].abs().max().item()
+
+    def run(prec, tol, roofline):
+        cfg = default_config(precision=prec, smoother=a.smoother, tol=tol, maxcy=200)
         s = PressureSolver(g, cfg)
         times, total, p = run_solves(torch, s, rhs, a.steps, a.warmup)
         st = s.stats
@@ -400,26 +507,20 @@ def main():
              "bytes_cycles": st.bytes_cycles,
              "cycle_GBps": st.bytes_cycles / (st.ms_cycles * 1e-3) / 1e9 if st.ms_cycles > 0 else None,
              "vcycles_per_s": st.iters / (st.ms_cycles * 1e-3) if st.ms_cycles > 0 else None,
-             "p50_ms": 1e3 * percentile(times, 50), "p99_ms": 1e3 * percentile(times, 99)}
-        if prec == "mixed":
+             "p50_ms": 1e3 * percentile(times, 50), "p99_ms": 1e3 * percentile(times, 99),
+             "pressure_err": (p[inner] - pt[inner]).abs().max().item() / pt_max}
+        if roofline:
             r["roofline"] = kernel_roofline(torch, s, a.kernel_reps)
             r["levels"] = s.levels
-        res[prec] = r
-        sol[prec] = p.clone()
-        print(f"# {prec}: {r['solves_per_s']:.1f} solves/s, {st.iters} cycles, "
-              f"relres {st.relres:.2e}", file=sys.stderr, flush=True)
+        print(f"# {prec} tol {tol:g}: {r['solves_per_s']:.1f} solves/s, {st.iters} cycles, "
+              f"relres {st.relres:.2e}, err {r['pressure_err']:.2e}", file=sys.stderr, flush=True)
         s.close()
+        return r
 
-    # pressure error of the mixed solve against a tight FP64 solve (null space removed by both)
-    cfg = default_config(precision="fp64", smoother=a.smoother, tol=1e-13, maxcy=200)
-    s = PressureSolver(g, cfg)
-    pt = torch.zeros_like(rhs)
-    s.mg_solver(pt, rhs)
-    tight_relres = s.stats.relres
-    s.close()
-    inner = (slice(1, -1),) * 3
-    err = {k: (sol[k][inner] - pt[inner]).abs().max().item() / pt[inner].abs().max().item()
-           for k in sol}
+    precs = ["mixed"] if a.no_fp64 else ["mixed", "fp64"]
+    res = {prec: run(prec, a.tol, prec == "mixed") for prec in precs}
+    res_w = {prec: run(prec, a.tol_workload, False) for prec in precs}
+    err = {k: res[k]["pressure_err"] for k in res}
 
     m = res["mixed"]
     line = {
@@ -439,6 +540,9 @@ def main():
                                "Neumann z" % a.beta,
                    "grid": list(n), "domain": [2 * math.pi, 2 * math.pi, 1.0],
                    "smoother": a.smoother, "cycle": "V(2,2)", "tol": a.tol,
+                   "stop": "relative residual <= tol, the loosest stop tested whose pressure "
+                           "is within 1e-9 of the tight FP64 solve (the workload's 1e-10 stop "
+                           "is timed under at_workload_tol)",
                    "precision": "FP32 V-cycle inside FP64 defect correction",
                    "levels": m["levels"]},
         "roofline": m["roofline"],
@@ -449,8 +553,16 @@ def main():
         "cycle_roofline_frac": (m["cycle_GBps"] / HBM_PEAK_GBPS) if m["cycle_GBps"] else None,
         "latency_ms": {"p50": m["p50_ms"], "p99": m["p99_ms"]},
         "pressure_err_vs_fp64_tight": err,
+        "parity_ok": all(v <= 1e-9 for v in err.values()),
         "fp64_tight_relres": tight_relres,
     }
+    wl = {"tol": a.tol_workload}
+    for k, r in res_w.items():
+        wl[k] = {q: r[q] for q in ("solves_per_s", "ms_per_solve", "iters", "relres",
+                                   "pressure_err", "p50_ms", "p99_ms")}
+    if "fp64" in res_w:
+        wl["speedup_mixed_vs_fp64"] = res_w["mixed"]["solves_per_s"] / res_w["fp64"]["solves_per_s"]
+    line["at_workload_tol"] = wl
     if "fp64" in res:
         f = res["fp64"]
         line["fp64"] = {k: f[k] for k in ("solves_per_s", "ms_per_solve", "iters", "relres",

@@ -263,6 +263,8 @@ int amg_server_drain(amg_server s);
  *         5: coarsest-level solve ('level' must be the coarsest)
  *         6: red half-sweep with a zero initial guess
  *         7/8: red/black z-line half-sweep of the coarse tail (short columns)
+ *         10/11: red/black z-line half-sweep in the short form (FP32)
+ *         12/13: red/black z-line half-sweep in the lean form
  * Returns the kernel's algorithmic bytes per launch.  The caller times the
  * stream (amg_stream) with its own events. */
 int amg_run_kernel(amg_handle h, int which, int level, int precision, int reps, double* bytes);

@@ -84,3 +84,26 @@ def test_dirichlet_closed_form_through_fortran(cuda, tmp_path, direction, precis
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.split()[:3] == ["PASS", direction, precision]
+
+
+def test_horizontally_varying_density_is_refused(tmp_path):
+    """The reference's Flow::r is 3-D (include/modules.F90:45); the shim
+    passes one column of it and refuses a density that varies horizontally
+    (AMG_EUNSUPPORTED from MG_init, before any device work)."""
+    r = _run("rho3d", str(tmp_path / "rho3d.ini"))
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split()[:2] == ["STAT", str(_capi.AMG_EUNSUPPORTED)]
+    assert "varies horizontally" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_relmax_stop_rule_through_fortran(cuda, tmp_path, precision):
+    """An INI with only tolmax (the reference's own stop rule,
+    mgMK/mg_solver.F90:114-128) converges on the relative solution change in
+    both precisions: FP64 computes the change although the Fortran caller
+    cannot ask for it."""
+    r = _run("relmax", precision, str(tmp_path / "relmax.ini"))
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.split()[:3] == ["PASS", "x3", precision]

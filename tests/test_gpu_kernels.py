@@ -155,3 +155,49 @@ def test_transfers_and_coarse_solve(cuda, case, precision):
     Minv = VC.coarse_operator_inverse(cl, is_singular(tbc))
     ref = (Minv @ fc.astype(np.float64).ravel()).reshape(fc.shape)
     assert relerr(got, ref) < (1e-11 if precision == "fp64" else 2e-6)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_zline_lean_form_full_columns(cuda, precision):
+    """The benchmarked z-line sweep: the finest level of the 256^3 stretched
+    workload (256-cell columns in 16 segments, a launch of 512 blocks, more
+    than the device's CUs, so it takes the lean form) against the oracle's
+    Thomas solve per column, both colours.  In FP32 the short form on the
+    same input gives the same bits; in either precision the lean form forced
+    onto a small launch gives the bits of that launch's own choice."""
+    from oracle import vcycle as VC
+    dt = np.float64 if precision == "fp64" else np.float32
+    s, g, levs, fac = _solver((256, 256, 256), 2.0, (0, 1, 0, 1, 0, 0), "LJ", precision)
+    lev = levs[0]
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal((256, 256, 256)).astype(dt)
+    f = rng.standard_normal((256, 256, 256)).astype(dt)
+    for which, parity in ((0, 0), (4, 1)):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        s.run_kernel(which, 0, 1)
+        got = s.level_field(0, 0)
+        ref = u.copy()
+        VC.zline_half_sweep(ref, f, lev, parity, dt)
+        err = relerr(got, ref)
+        print(f"256^3 {precision} z-line parity {parity}: err {err:.2e}", flush=True)
+        assert err < _tol(precision, "LJ"), (parity, err)
+        if precision == "mixed":
+            s.level_field(0, 0, u)
+            s.run_kernel(10 + parity, 0, 1)
+            assert np.array_equal(s.level_field(0, 0), got), parity
+    s.close()
+    # a small launch (16x16 columns of 64 cells: 2 blocks) takes the short
+    # form in FP32; the lean form forced onto it gives the same bits
+    s, g, levs, fac = _solver((16, 16, 64), 1.0, (0, 1, 0, 1, 0, 0), "LJ", precision)
+    u = rng.standard_normal((64, 16, 16)).astype(dt)
+    f = rng.standard_normal((64, 16, 16)).astype(dt)
+    for parity in (0, 1):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        s.run_kernel(4 * parity, 0, 1)
+        auto = s.level_field(0, 0)
+        s.level_field(0, 0, u)
+        s.run_kernel(12 + parity, 0, 1)
+        assert np.array_equal(s.level_field(0, 0), auto), parity
+    s.close()
