@@ -103,6 +103,8 @@ _PROTOS = {
                                    C.c_double]),
     "amg_divergence_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]),
+    "amg_divergence_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, _dp, _dp, C.POINTER(C.c_int)]),
     "amg_ghost_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p, C.POINTER(amg_flowbc)]),
     "amg_pgrad_ghost_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_double, C.POINTER(amg_flowbc)]),

@@ -180,6 +180,18 @@ int amg_pgrad_device(amg_handle h, const double* p, double* U, double* V, double
 int amg_divergence_device(amg_handle h, const double* U, const double* V, const double* W,
                           double* div_or_null, double* avg, double* maxabs, int loc[3]);
 
+/* Allocation-free divergence report: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * kernels and results as amg_divergence_device, which allocates its block
+ * partials on its first call; this one keeps them in a static 256 KiB
+ * device array per GPU instead and allocates nothing.  Limit: grids of at
+ * most 33 554 432 cells (AMG_EBADSIZE beyond).  Calls that overlap on one
+ * GPU share that array and race. */
+int amg_divergence_device_noalloc(amg_handle h, const double* U, const double* V,
+                                  const double* W, double* div_or_null, double* avg,
+                                  double* maxabs, int loc[3]);
+
 /* ---- the time stepper's boundary handling and vertical viscous step -------
  * Boundary conditions of the flow fields, the reference's module boundC
  * (include/modules.F90:82-85): TB(1:3,1:2,1:12) (type) and VB(1:3,1:2,1:12)
