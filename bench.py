@@ -396,26 +396,26 @@ def _mgmk_time(n, levels, procs, maxcy, count, tmp, rhsfile):
     return float(row[3]), float(row[5])
 
 
-def cpu_baseline_reference(n=256, levels=7):
+def cpu_baseline_reference(n=256):
     """The reference's own CPU solver (mgMK built from its Fortran sources,
     oracle/_ref/mgmk_harness) on this host's cores: the uniform n^3
     analogue of the workload (2*pi x 2*pi x 1, periodic x/y, Neumann z;
     mgMK's operator is the uniform one, so it cannot take the stretched
-    grid), RHS = Psource of N(0,1) face velocities, RBPGS V(2,2), MPI ranks
-    split over x and y.  Bounded sample: mg_solver calls of 2 and 8
-    V-cycles, timed with MPI_WTIME around the call; a solve to a relative
-    residual of 1e-10 costs the 2-cycle call plus the marginal cycle time
-    for the rest of the cycles the reference needs (tests/golden/
-    mgmk_cycles.json, measured with the same build)."""
+    grid), RHS = Psource of N(0,1) face velocities, RBPGS V(2,2), with the
+    level count and the x/y split of MPI ranks of tests/golden/
+    mgmk_cycles.json (the splits the reference computes correctly).
+    Bounded sample: mg_solver calls of 2 and 8 V-cycles, timed with
+    MPI_WTIME around the call; a solve to a relative residual of 1e-10
+    costs the 2-cycle call plus the marginal cycle time for the rest of the
+    cycles the reference needs (the same file, measured with this build)."""
     import tempfile
     import numpy as np
     from anelastic_solve_amd import workloads
     from oracle import grid as G, ops
     with open(MGMK_CYCLES) as fh:
-        need = json.load(fh)["sizes"][str(n)]["cycles_to_1e-10"]
-    cores = min(16, os.cpu_count() or 1)
-    px = 4 if cores >= 4 else cores
-    py = max(1, min(4, cores // px))
+        rec = json.load(fh)["sizes"][str(n)]
+    need, levels = rec["cycles_to_1e-10"], rec["levels"]
+    px, py, _ = rec["ranks"]
     with tempfile.TemporaryDirectory() as tmp:
         g, U, V, W = workloads.make(workloads.UNIFORM_64, n=(n, n, n))
         b = workloads.psource_host(U, V, W, g)
