@@ -49,6 +49,7 @@ class amg_config(C.Structure):
         ("verbose", C.c_int),
         ("strict", C.c_int),
         ("use_graph", C.c_int),
+        ("coarse_direct", C.c_int),
     ]
 
 
@@ -97,6 +98,7 @@ _PROTOS = {
                                    C.POINTER(amg_stats)]),
     "amg_stream": (C.c_void_p, [C.c_void_p]),
     "amg_levels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int * 3)]),
+    "amg_direct_level": (C.c_int, [C.c_void_p]),
     "amg_psource_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_double]),
     "amg_pgrad_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

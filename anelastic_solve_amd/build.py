@@ -13,8 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libanelastic_mg.so")
 
-SOURCES = ["amg_kernels.hip", "amg_flow.hip", "amg_hierarchy.cpp", "amg_solver.cpp",
-           "amg_server.cpp", "amg_capi.cpp", "amg_noalloc.hip"]
+SOURCES = ["amg_kernels.hip", "amg_flow.hip", "amg_hierarchy.cpp", "amg_direct.cpp",
+           "amg_solver.cpp", "amg_server.cpp", "amg_capi.cpp", "amg_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")

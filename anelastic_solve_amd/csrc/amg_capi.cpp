@@ -168,6 +168,7 @@ void amg_config_default(amg_config* c) {
   c->precision = AMG_MIXED;
   c->report_relmax = 0;
   c->use_graph = 1;
+  c->coarse_direct = 1;
 }
 
 // mgMK/mg_read_params.F90:44-214
@@ -201,7 +202,9 @@ int amg_read_params(const char* infile, amg_config* cfg) {
     else { set_error(std::string("ERROR: ") + key + " MUST BE yes/no READ: " + s); return false; }
     return true;
   };
-  if (!logical("Strict", c.strict) || !logical("Verbose", c.verbose)) return AMG_EPARSE;
+  if (!logical("Strict", c.strict) || !logical("Verbose", c.verbose) ||
+      !logical("CoarseDirect", c.coarse_direct))
+    return AMG_EPARSE;
   const char* lk[3] = {"nxlevels", "nylevels", "nzlevels"};
   for (int d = 0; d < 3; ++d) {
     if (!need("MGsolver", lk[d], v)) return AMG_EPARSE;
@@ -319,6 +322,8 @@ int amg_levels(amg_handle h, int* nlevels, int sizes[][3]) {
     }
   return AMG_OK;
 }
+
+int amg_direct_level(amg_handle h) { return h ? h->I->ld : -1; }
 
 static int finish_solve(amg_handle h, int rc, const amg_stats& st, int* iter, double* relmax) {
   if (iter) *iter = st.iters;

@@ -49,6 +49,10 @@ struct DevLevel {
   void* u2;                  // FP64 finest: ping-pong partner of u
   void* f;                   // right-hand side (chequer layout)
   void* scr;                 // z-line scratch: nz * hx * ny values
+  // separable direct solve of a coarse level (amg_direct.cpp, k_direct_*)
+  const double* dmat;        // [Vx^-T | Vx^T | Vy^-1 | Vy | Vz^-1 | Vz], FP64 (z padded to 16)
+  const double* dinv;        // 1 / (lx + ly + lz) per eigenmode, [k][g][e]
+  double* dscr;              // transformed right-hand side, then solution: 2 * nz*ny*nx (mode fastest)
 };
 
 // One kernel launch over several items (levels of different instances),
@@ -199,6 +203,19 @@ void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const La
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s);
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
+// Separable direct solve of the items' levels (u = A^-1 f), for levels with
+// nx, ny multiples of 16 and nz <= 64.  Every pass splits its output into
+// slabs of kDirectCols columns, one workgroup each (a level of 64^3 fills
+// the device with 256): 'planes' has nz*nx/kDirectCols blocks per item,
+// 'slabs' nx*ny/kDirectCols; lds = the largest direct_lds() of the items,
+// at most direct_lds_limit().
+constexpr long long kDirectCells = 1 << 18;
+constexpr int kDirectCols = 16;
+constexpr int kDirectMaxNz = 64;
+size_t direct_lds(int nx, int ny, int nz);
+size_t direct_lds_limit();
+void launch_direct(bool fp64, const DevLevel* L, const LaunchList& planes, const LaunchList& slabs,
+                   size_t lds, hipStream_t s);
 // the coarse tail's short-column z-line sweep as a standalone launch (tests)
 void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                       int zero_guess, hipStream_t s);

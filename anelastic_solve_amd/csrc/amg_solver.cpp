@@ -7,6 +7,8 @@
 // is one launch over all instances of a set that take part in it.
 #include "amg_solver.h"
 
+#include "amg_direct.h"
+
 #include <algorithm>
 #include <cstddef>
 #include <cmath>
@@ -83,8 +85,12 @@ size_t cycle_bytes(const Instance& I) {
   const double w = I.fp64 ? 8.0 : 4.0;
   double b = 0.0;
   const int L = (int)I.H.lv.size();
-  for (int l = 0; l + 1 < L; ++l)
+  // levels with smoothing and transfers: above the direct level, or every
+  // level but the coarsest; a direct level reads f and writes u
+  const int top = I.ld >= 0 ? I.ld : L - 1;
+  for (int l = 0; l < top; ++l)
     b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + 4.25);
+  if (I.ld >= 0) b += 2.0 * w * (double)I.H.lv[I.ld].ncell();
   const double n0 = (double)I.H.lv[0].ncell();
   b += I.fp64 ? 16.0 * n0 : 32.0 * n0;
   return (size_t)b;
@@ -445,6 +451,36 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     I->tail = (int)hT.size();
     I->tail_lds = off;
   }
+  // separable direct solve of the first level below the finest with at most
+  // kDirectCells cells, when that level lies above the coarse tail (the
+  // tail solves small levels in one launch); the levels below it are not
+  // cycled
+  if (cfg.coarse_direct != 0) {
+    int cand = -1;
+    for (int l = 1; l < L; ++l)
+      if (I->H.lv[l].ncell() <= kDirectCells) { cand = l; break; }
+    if (cand >= 1 && cand < I->l0) {
+      const HostLevel& H = I->H.lv[cand];
+      const size_t lds = direct_lds(H.nx, H.ny, H.nz);
+      DirectTables D;
+      std::string derr;
+      if (H.nx % 16 == 0 && H.ny % 16 == 0 && H.nz <= kDirectMaxNz && lds <= direct_lds_limit() &&
+          direct_setup(H, I->H.bc, I->H.singular, &D, &derr)) {
+        double* dm = (double*)I->mem.alloc(D.mats.size() * 8);
+        double* di = (double*)I->mem.alloc(D.inv.size() * 8);
+        double* ds = (double*)I->mem.alloc(2 * (size_t)H.ncell() * 8);
+        if (!dm || !di || !ds) return fail();
+        AMG_HIP(hipMemcpy(dm, D.mats.data(), D.mats.size() * 8, hipMemcpyHostToDevice));
+        AMG_HIP(hipMemcpy(di, D.inv.data(), D.inv.size() * 8, hipMemcpyHostToDevice));
+        DevLevel& d = newL[cand];
+        d.dmat = dm;
+        d.dinv = di;
+        d.dscr = ds;
+        I->ld = cand;
+        I->dlds = lds;
+      }
+    }
+  }
   // projection metrics
   {
     const HostLevel& F = I->H.lv[0];
@@ -528,12 +564,33 @@ void Context::zline_launches(const std::vector<Item>& items, bool fp64, int pari
       });
 }
 
+// Direct solves: the plane and mode lists hold the same levels in the same
+// order, launched in chunks of at most kMaxItems levels.
+void Context::direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes,
+                              size_t lds, bool fp64) {
+  for (size_t q0 = 0; q0 < planes.size(); q0 += kMaxItems) {
+    LaunchList lp, lm;
+    lp.nitems = lm.nitems = 0;
+    lp.nblocks = lm.nblocks = 0;
+    lp.prefix[0] = lm.prefix[0] = 0;
+    for (size_t q = q0; q < std::min(planes.size(), q0 + kMaxItems); ++q) {
+      lp.items[lp.nitems] = planes[q].idx;
+      lp.nblocks += planes[q].blocks;
+      lp.prefix[++lp.nitems] = lp.nblocks;
+      lm.items[lm.nitems] = modes[q].idx;
+      lm.nblocks += modes[q].blocks;
+      lm.prefix[++lm.nitems] = lm.nblocks;
+    }
+    launch_direct(fp64, dL, lp, lm, lds, stream);
+  }
+}
+
 // ---------------------------------------------------------------------------
 void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   if (S.empty()) return;
   const bool zline = cfg.smoother == AMG_SMOOTHER_LJ;
-  int maxL0 = 0;   // levels above the coarse tail, cycled with one launch per step
-  for (Instance* I : S) maxL0 = std::max(maxL0, I->l0);
+  int maxL0 = 0;   // levels above the direct level / coarse tail, one launch per step
+  for (Instance* I : S) maxL0 = std::max(maxL0, I->lc());
   const DevLevel* L = dL;
   auto half = [&](const std::vector<Item>& items, int parity, int zg) {
     if (zline) zline_launches(items, fp64, parity, zg);
@@ -547,11 +604,11 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   auto smooth = [&](int dpt, bool pre, bool zero_first) {
     int maxs = 0;
     for (Instance* I : S)
-      if (dpt < I->l0) maxs = std::max(maxs, pre ? I->pre[dpt] : I->post[dpt]);
+      if (dpt < I->lc()) maxs = std::max(maxs, pre ? I->pre[dpt] : I->post[dpt]);
     for (int s = 0; s < maxs; ++s) {
       std::vector<Item> items;
       for (Instance* I : S) {
-        if (dpt >= I->l0) continue;
+        if (dpt >= I->lc()) continue;
         if (s >= (pre ? I->pre[dpt] : I->post[dpt])) continue;
         const int li = I->lev[dpt];
         items.push_back({li, blocks_smooth(hL[li])});
@@ -566,13 +623,13 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
     const bool zero_first = (dpt > 0) || !fp64;
     std::vector<Item> zero;
     for (Instance* I : S)
-      if (dpt < I->l0 && zero_first && I->pre[dpt] == 0)
+      if (dpt < I->lc() && zero_first && I->pre[dpt] == 0)
         zero.push_back({I->lev[dpt], cdiv(hL[I->lev[dpt]].ncell, kBlock)});
     for_lists(zero, [&](const LaunchList& ll, int, int) { launch_fill_zero(fp64, L, ll, stream); });
     smooth(dpt, true, zero_first);
     std::vector<Item> items, items222;
     for (Instance* I : S) {
-      if (dpt >= I->l0) continue;
+      if (dpt >= I->lc()) continue;
       const int xi = I->xf[dpt];
       const DevLevel& C = hL[I->lev[dpt + 1]];
       if (restrict222_ok(hL[I->lev[dpt]], C, hX[xi])) items222.push_back({xi, restrict222_blocks(C)});
@@ -585,6 +642,19 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
       launch_restrict(fp64, L, dX, ll, stream);
     });
   }
+  // the direct levels: transform, z solves, transform back
+  {
+    std::vector<Item> planes, modes;
+    size_t lds = 0;
+    for (Instance* I : S) {
+      if (I->ld < 0) continue;
+      const int li = I->lev[I->ld];
+      planes.push_back({li, hL[li].nz * (hL[li].nx / kDirectCols)});
+      modes.push_back({li, hL[li].ny * (hL[li].nx / kDirectCols)});
+      lds = std::max(lds, I->dlds);
+    }
+    direct_launches(planes, modes, lds, fp64);
+  }
   // the coarse tails, one launch per kernel variant; a split tail runs down,
   // then the coarse solve across the device, then up
   for (int nzc : {0, 8, 16, 32}) {
@@ -593,6 +663,7 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
         std::vector<Item> items, coarse;
         size_t lds = 0;
         for (Instance* I : S) {
+          if (I->ld >= 0) continue;
           if (I->tail_nzc != nzc || (int)I->tail_222 != a222 || (int)I->tail_split != split) continue;
           items.push_back({I->tail, 1});
           lds = std::max(lds, I->tail_lds);
@@ -621,7 +692,7 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   for (int dpt = maxL0 - 1; dpt >= 0; --dpt) {
     std::vector<Item> items;
     for (Instance* I : S)
-      if (dpt < I->l0)
+      if (dpt < I->lc())
         items.push_back({I->xf[dpt], cdiv(prolong_duals(hL[I->lev[dpt]], hL[I->lev[dpt + 1]],
                                                         hX[I->xf[dpt]]),
                                           kBlock)});
@@ -1067,6 +1138,18 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     }
     items.push_back({I->lev[level], cdiv((long long)d.hx * d.ny, kBlock)});
     b = 1.5 * w * d.ncell;
+  } else if (which == 14) {
+    if (level != I->ld) {
+      set_error("amg_run_kernel: the direct solve runs on the direct level");
+      return AMG_EINVAL;
+    }
+    b = 2.0 * w * d.ncell;
+    for (int r = 0; r < reps; ++r)
+      direct_launches({{I->lev[level], d.nz * (d.nx / kDirectCols)}},
+                      {{I->lev[level], d.ny * (d.nx / kDirectCols)}}, I->dlds, fp64);
+    AMG_HIP(hipGetLastError());
+    if (bytes) *bytes = b;
+    return AMG_OK;
   } else if (which == 5) {
     if (level != L - 1) {
       set_error("amg_run_kernel: the coarse solve runs on the coarsest level");

@@ -51,6 +51,10 @@ struct Instance {
   std::vector<int> lev;          // DevLevel index per level, finest first
   std::vector<int> xf;           // DevXfer index per transition
   int l0 = 0;                    // first level of the coarse tail
+  int ld = -1;                   // level solved directly (separable solve), or -1
+  size_t dlds = 0;               // dynamic LDS of its plane transforms
+  // the first level not cycled with one launch per step
+  int lc() const { return ld >= 0 ? ld : l0; }
   int tail = -1;                 // DevTail index
   size_t tail_lds = 0;           // dynamic LDS bytes of the tail
   int tail_nzc = 16;             // tail kernel variant: column length class
@@ -111,6 +115,8 @@ struct Context {
   // pieces
   void vcycle(const std::vector<Instance*>& S, bool fp64);
   void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg, int form = 0);
+  void direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes, size_t lds,
+                       bool fp64);
   int prepare(const std::vector<Instance*>& S);
   const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);

@@ -32,6 +32,7 @@ module amg_cabi
      integer(c_int) :: verbose
      integer(c_int) :: strict
      integer(c_int) :: use_graph
+     integer(c_int) :: coarse_direct
   end type amg_config
 
   type, bind(C) :: amg_grid

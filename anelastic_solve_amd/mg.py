@@ -30,7 +30,7 @@ def default_config(**kw):
     'relmax'), pre, post (int or coarsest-first list), tbc (6 ints, face
     order X2max, X3min, X2min, X3max, X1min, X1max), vbc (6 floats),
     precision ('fp64'|'mixed'), nlevels (3 ints), report_relmax, verbose,
-    use_graph.
+    use_graph, coarse_direct.
     """
     lib = _capi.load()
     c = _capi.amg_config()
@@ -56,7 +56,7 @@ def default_config(**kw):
         elif k == "nlevels":
             for i in range(3):
                 c.nlevels[i] = int(v[i])
-        elif k in ("maxcy", "report_relmax", "verbose", "use_graph", "strict"):
+        elif k in ("maxcy", "report_relmax", "verbose", "use_graph", "strict", "coarse_direct"):
             setattr(c, k, int(v))
         elif k == "tol":
             c.tol = float(v)
@@ -214,6 +214,11 @@ class PressureSolver:
         sizes = ((C.c_int * 3) * _capi.MAX_LEVELS)()
         check(self._lib.amg_levels(self._h, C.byref(n), sizes), "amg_levels")
         return [tuple(sizes[l]) for l in range(n.value)]
+
+    @property
+    def direct_level(self):
+        """The level solved by the separable direct solve, or -1."""
+        return self._lib.amg_direct_level(self._h)
 
     @property
     def stream(self):

@@ -7,10 +7,11 @@ the mid-plane), RHS = divergence of a seeded random velocity with a k^-5/3
 spectrum in x/y.  A step is one solve from a zero initial guess with
 V(2,2) cycles and the z-line red-black Gauss-Seidel smoother, inputs
 resident in HBM, to a relative residual ||b - A p||_inf / ||b||_inf of
-1e-12: at the workload's 1e-10 the pressure is still 9e-9 (relative,
-max-norm) from the exact answer on this stretched grid, and 1e-12 is the
-stop at which it is within the 1e-9 bar.  The 1e-10 rates are reported
-beside it (at_workload_tol).
+1e-10 (the workload's stop) when the pressure there is within 1e-9
+(relative, max-norm) of a tight FP64 solve in both precisions, else to
+1e-12 (the stop that holds the bar on this stretched grid whatever the
+cycle count).  Both stops are timed; the JSON line names the one taken
+(config.tol) and carries the other beside it.
 
 ``value`` is solves/s of the FP32-smoothed / FP64-corrected solver; the
 plain FP64 V-cycle runs on the same input for the comparison.  The JSON
@@ -42,10 +43,10 @@ def parse():
     ap.add_argument("--n", type=int, nargs=3, default=[256, 256, 256])
     ap.add_argument("--beta", type=float, default=2.0)
     ap.add_argument("--tol", type=float, default=None,
-                    help="the parity stop: the relative residual at which the pressure is "
-                         "within 1e-9 of the tight FP64 solve (value is measured here)")
+                    help="the stop of the headline rate (default: the workload's stop when "
+                         "its pressure is within 1e-9 of the tight FP64 solve, else 1e-12)")
     ap.add_argument("--tol-workload", type=float, default=1e-10,
-                    help="the workload's stated stop, timed as a secondary rate")
+                    help="the workload's stated stop")
     ap.add_argument("--smoother", default="LJ")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -478,8 +479,6 @@ def main():
     import torch
     from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
 
-    if a.tol is None:
-        a.tol = workloads.PARITY_TOL
     n = tuple(a.n)
     g = box_grid(*n, beta=a.beta)
     U, V, W = workloads.velocity_spectral(g, 1)
@@ -518,8 +517,19 @@ def main():
         return r
 
     precs = ["mixed"] if a.no_fp64 else ["mixed", "fp64"]
-    res = {prec: run(prec, a.tol, prec == "mixed") for prec in precs}
-    res_w = {prec: run(prec, a.tol_workload, False) for prec in precs}
+    res_w = {prec: run(prec, a.tol_workload, prec == "mixed") for prec in precs}
+    res_p = {prec: run(prec, workloads.PARITY_TOL, prec == "mixed") for prec in precs}
+    if a.tol is None:
+        # the loosest stop whose pressure holds the 1e-9 bar in every precision
+        ok_w = all(r["pressure_err"] <= 1e-9 for r in res_w.values())
+        a.tol = a.tol_workload if ok_w else workloads.PARITY_TOL
+    if a.tol == a.tol_workload:
+        res, other, other_tol = res_w, res_p, workloads.PARITY_TOL
+    elif a.tol == workloads.PARITY_TOL:
+        res, other, other_tol = res_p, res_w, a.tol_workload
+    else:
+        res = {prec: run(prec, a.tol, prec == "mixed") for prec in precs}
+        other, other_tol = res_w, a.tol_workload
     err = {k: res[k]["pressure_err"] for k in res}
 
     m = res["mixed"]
@@ -540,9 +550,9 @@ def main():
                                "Neumann z" % a.beta,
                    "grid": list(n), "domain": [2 * math.pi, 2 * math.pi, 1.0],
                    "smoother": a.smoother, "cycle": "V(2,2)", "tol": a.tol,
-                   "stop": "relative residual <= tol, the loosest stop tested whose pressure "
-                           "is within 1e-9 of the tight FP64 solve (the workload's 1e-10 stop "
-                           "is timed under at_workload_tol)",
+                   "stop": "relative residual <= tol: the workload's 1e-10 when the pressure "
+                           "there is within 1e-9 of the tight FP64 solve in both precisions, "
+                           "else 1e-12 (the other stop is timed under other_tol)",
                    "precision": "FP32 V-cycle inside FP64 defect correction",
                    "levels": m["levels"]},
         "roofline": m["roofline"],
@@ -556,13 +566,13 @@ def main():
         "parity_ok": all(v <= 1e-9 for v in err.values()),
         "fp64_tight_relres": tight_relres,
     }
-    wl = {"tol": a.tol_workload}
-    for k, r in res_w.items():
+    wl = {"tol": other_tol}
+    for k, r in other.items():
         wl[k] = {q: r[q] for q in ("solves_per_s", "ms_per_solve", "iters", "relres",
                                    "pressure_err", "p50_ms", "p99_ms")}
-    if "fp64" in res_w:
-        wl["speedup_mixed_vs_fp64"] = res_w["mixed"]["solves_per_s"] / res_w["fp64"]["solves_per_s"]
-    line["at_workload_tol"] = wl
+    if "fp64" in other:
+        wl["speedup_mixed_vs_fp64"] = other["mixed"]["solves_per_s"] / other["fp64"]["solves_per_s"]
+    line["other_tol"] = wl
     if "fp64" in res:
         f = res["fp64"]
         line["fp64"] = {k: f[k] for k in ("solves_per_s", "ms_per_solve", "iters", "relres",

@@ -87,6 +87,11 @@ typedef struct amg_config {
   int    verbose;               /* print one line per V-cycle to stderr               */
   int    strict;                /* accepted for compatibility; one GPU has no halos   */
   int    use_graph;             /* capture the V-cycle into a hipGraph (default 1)    */
+  int    coarse_direct;         /* 1 (default): solve the first level below the finest
+                                   with at most 2^18 cells exactly (separable solve in
+                                   FP64) when z is not periodic and the level is above
+                                   the coarse tail; the levels below it are not cycled.
+                                   0: cycle every level down to the coarsest           */
 } amg_config;
 
 /*
@@ -166,6 +171,10 @@ void* amg_stream(amg_handle h);
 
 /* Number of levels and their sizes (finest first). */
 int amg_levels(amg_handle h, int* nlevels, int sizes[][3]);
+
+/* The level solved directly by the separable solve (amg_config.coarse_direct),
+ * or -1 when the V-cycle goes down to the coarsest level. */
+int amg_direct_level(amg_handle h);
 
 /* ---- projection step (FP64, device arrays, reference layout) ---------------
  * Psource: SRC = D(rho u*)/dt on interior cells, ghosts zeroed
@@ -277,6 +286,7 @@ int amg_server_drain(amg_server s);
  *         7/8: red/black z-line half-sweep of the coarse tail (short columns)
  *         10/11: red/black z-line half-sweep in the short form (FP32)
  *         12/13: red/black z-line half-sweep in the lean form
+ *         14: separable direct solve u = A^-1 f ('level' must be amg_direct_level)
  * Returns the kernel's algorithmic bytes per launch.  The caller times the
  * stream (amg_stream) with its own events. */
 int amg_run_kernel(amg_handle h, int which, int level, int precision, int reps, double* bytes);

@@ -212,12 +212,86 @@ def coarse_operator_inverse(lev, singular):
     return np.linalg.inv(A)
 
 
+def separable_solve(lev, f, singular):
+    """Exact solve of ``A x = f`` on one level with non-periodic z (FP64).
+
+    ``A`` is a sum of 1-D operators; ``Ax = Dx^-1 Sx`` (``Dx`` the cell
+    widths, ``Sx`` symmetric), so ``Dx^1/2 Ax Dx^-1/2 = Q diag(l) Q^T`` and
+    ``Ax = Vx diag(l) Vx^-1`` with ``Vx = Dx^-1/2 Q``; the same in y.  In
+    the eigenvector bases every horizontal mode ``(e, g)`` is one tridiagonal
+    system ``(Az + lx_e + ly_g) x = f^`` in z.  A singular operator's
+    constant mode has its last value pinned to zero, and the result is made
+    orthogonal to the null vector ``rho(z)`` (volume weights), as
+    ``ops.null_project`` does for the whole level.  This is the coarse
+    solve of a direct level (the product's ``k_direct_*``).
+    """
+    nx, ny, nz = lev["n"]
+    px, py, pz = lev["per"]
+    assert not pz, "separable_solve needs walls in z"
+
+    def axis(c, dxc, n, per):
+        wm, wp, d = (np.asarray(a, dtype=np.float64) for a in c[:3])
+        A = np.diag(d)
+        for i in range(n):
+            if per or i > 0:
+                A[i, (i - 1) % n] += wm[i]
+            if per or i < n - 1:
+                A[i, (i + 1) % n] += wp[i]
+        s = np.sqrt(dxc)
+        B = s[:, None] * A / s[None, :]
+        lam, Q = np.linalg.eigh(0.5 * (B + B.T))
+        return lam, Q / s[:, None], Q.T * s[None, :]
+
+    dxc = np.asarray(lev["ax"]["dxc"][1:nx + 1])
+    dyc = np.asarray(lev["ay"]["dxc"][1:ny + 1])
+    dzc = np.asarray(lev["az"]["dxc"][1:nz + 1])
+    lx, Vx, Vxi = axis(lev["cx"], dxc, nx, px)
+    ly, Vy, Vyi = axis(lev["cy"], dyc, ny, py)
+    sing = None
+    if singular:
+        ix, iy = int(np.argmin(np.abs(lx))), int(np.argmin(np.abs(ly)))
+        lx[ix] = 0.0
+        ly[iy] = 0.0
+        sing = (iy, ix)
+    fh = np.einsum("ei,kji->kje", Vxi, np.asarray(f, dtype=np.float64))
+    fh = np.einsum("gj,kje->kge", Vyi, fh)
+    wzm, wzp, dz = (np.asarray(a, dtype=np.float64) for a in lev["cz"][:3])
+    lam = ly[:, None] + lx[None, :]
+
+    def thomas(g, lm, rows):
+        x = np.zeros_like(g)
+        cp = np.zeros_like(g)
+        y = np.zeros_like(g)
+        for k in range(rows):
+            a = wzm[k] if k > 0 else 0.0
+            c = wzp[k] if k < nz - 1 else 0.0
+            r = 1.0 / ((dz[k] + lm) - a * (cp[k - 1] if k > 0 else 0.0))
+            y[k] = (g[k] - a * (y[k - 1] if k > 0 else 0.0)) * r
+            cp[k] = c * r
+        for k in range(rows - 1, -1, -1):
+            x[k] = y[k] - (cp[k] * x[k + 1] if k + 1 < nz else 0.0)
+        return x
+
+    xh = thomas(fh, lam[None, :, :], nz)
+    if sing is not None:
+        g0 = fh[:, sing[0], sing[1]]
+        x0 = thomas(g0, 0.0, nz - 1)
+        rho = np.asarray(lev["rho"][1:nz + 1], dtype=np.float64)
+        xh[:, sing[0], sing[1]] = x0 - ((dzc * x0).sum() / (dzc * rho).sum()) * rho
+    x = np.einsum("jg,kge->kje", Vy, xh)
+    return np.einsum("ie,kje->kji", Vx, x)
+
+
 def vcycle(levels, factors, u, f, pre, post, smoother, dt, Minv, l=0, zero_guess=False):
     """One V-cycle on level ``l`` (``mgMK/mg_cycle.F90``): pre-smooth, restrict,
-    recurse, prolongation-add, post-smooth; exact solve on the coarsest level."""
+    recurse, prolongation-add, post-smooth; exact solve on the coarsest level
+    (``Minv`` a dense inverse, or a function solving that level)."""
     lev = levels[l]
     if l == len(levels) - 1:
-        x = (Minv @ f.astype(np.float64).ravel()).reshape(f.shape)
+        if callable(Minv):
+            x = Minv(f.astype(np.float64))
+        else:
+            x = (Minv @ f.astype(np.float64).ravel()).reshape(f.shape)
         u[...] = x.astype(dt)
         return u
     if zero_guess:
@@ -237,7 +311,7 @@ def maxabs(a):
 
 
 def solve(levels, factors, tbc, vbc, p0, b_in, *, pre=2, post=2, smoother=LJ,
-          mixed=False, tol=1e-10, maxcy=200, remove_null=True):
+          mixed=False, tol=1e-10, maxcy=200, remove_null=True, direct_level=None):
     """Solve to ``||b - A p||_inf / ||b||_inf <= tol`` (driver of ``mgMK/mg_solver.F90``).
 
     ``mixed=False``: V-cycles on ``p`` in FP64.  ``mixed=True``: FP64 defect
@@ -245,16 +319,24 @@ def solve(levels, factors, tbc, vbc, p0, b_in, *, pre=2, post=2, smoother=LJ,
     ``e = 0``, ``p += e`` (FP64).  Returns ``(p, info)``; ``info`` holds the
     cycle count, the residual history and the reference's ``relmax =
     max|p_new - p_old| / max|p_new|`` history (``mgMK/mg_residual.F90:60-94``).
+    ``direct_level``: cycle the levels above it and solve it exactly
+    (``separable_solve``) instead of going down to the coarsest level.
     """
+    singular = is_singular(tbc)
+    if direct_level is not None:
+        levels = levels[:direct_level + 1]
+        factors = factors[:direct_level]
     L = len(levels)
     fine = levels[0]
     pre_l = [pre] * L if np.isscalar(pre) else list(pre)
     post_l = [post] * L if np.isscalar(post) else list(post)
-    singular = is_singular(tbc)
     b = fold_bc_rhs(b_in, fine, tbc, vbc)
     if singular:
         b = compat_project(b, fine)
-    Minv = coarse_operator_inverse(levels[-1], singular)
+    if direct_level is not None:
+        Minv = lambda g: separable_solve(levels[-1], g, singular)  # noqa: E731
+    else:
+        Minv = coarse_operator_inverse(levels[-1], singular)
     p = np.array(p0, dtype=np.float64, copy=True)
     bn = maxabs(b)
     res_hist, rel_hist = [], []

@@ -2,8 +2,8 @@
 
 * uniform 64^3, plain FP64, point red-black GS V(2,2), white-noise RHS;
 * stretched 256^3 (tanh, beta 2), k^-5/3 RHS, mixed and FP64 against the
-  oracle's exact FFT(x,y)+Thomas(z) solve at the measured stop
-  (``workloads.PARITY_TOL``), and the workload's own 1e-10 stop reached;
+  oracle's exact FFT(x,y)+Thomas(z) solve at the workload's 1e-10 stop and
+  at ``workloads.PARITY_TOL``;
 * the shear-layer projection sequence at 512x256x256: all 60 warm-started
   solves reach 1e-10, each projected velocity's divergence drops by the same
   factor, and one more substep's pressure matches the oracle;
@@ -59,9 +59,9 @@ def test_uniform64_fp64_point_rbgs(cuda):
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_stretched256_against_oracle(cuda, precision):
-    """At the stop the benchmark measures (PARITY_TOL) the pressure is within
-    1e-9 of the oracle; the workload's 1e-10 stop is reached too (its error is
-    printed: the grid's error-to-residual ratio is about 300)."""
+    """At both stops the benchmark measures, the workload's 1e-10 (its
+    headline when the pressure there holds the bar) and PARITY_TOL, the
+    pressure is within 1e-9 of the oracle."""
     from anelastic_solve_amd import workloads
     g, U, V, W = workloads.make(workloads.STRETCHED_256)
     b = workloads.psource_host(U, V, W, g)
@@ -72,7 +72,7 @@ def test_stretched256_against_oracle(cuda, precision):
     e12 = relerr(got, x)
     print(f"256^3 {precision}: tol 1e-10 -> {it} cycles, err {e10:.2e}; "
           f"tol {workloads.PARITY_TOL:g} -> {it2} cycles, err {e12:.2e}")
-    assert ok and rr <= 1e-10
+    assert ok and rr <= 1e-10 and e10 <= 1e-9
     assert ok2 and rr2 <= workloads.PARITY_TOL and e12 <= 1e-9
 
 
