@@ -31,6 +31,7 @@ struct DevLevel {
   int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
   int nullsp;                // the operator is singular (periodic/Neumann): constants in the null space
   int stop;                  // gated cycles: set once the stop rule held; the kernels then return
+  int gslot;                 // the instance's slot in the per-instance gate array
   unsigned mhx, mny;         // ceil(2^32/hx), ceil(2^32/ny) (0 for a divisor of 1): e/hx = umulhi(e, mhx)
                              // exactly while e*hx < 2^32 (the coarse tail's small levels)
   unsigned gmhx, gmny;       // division of any 32-bit index by hx, ny: multiplier and
@@ -81,6 +82,23 @@ struct CycleGate {
   int maxcy, rule, fp64, report, pad;
   double bnorm, tol;
   double hist[kGateHist][6];   // per cycle: the 4 norms of the cycle, and the 2 of relmax (FP64)
+};
+
+// Gated cycles of a batch of instances (served rounds): one gate per
+// instance, in one array indexed by DevLevel::gslot, so that the host reads
+// every gate of a round with one copy.  Each cycle's last norm fold records
+// the cycle's norms in the instance's ring (entry count % kItemRing) and
+// applies the stop rule of the host's norm step; a stop sets DevLevel::stop
+// of the instance's levels.  The gate is initialised on the device from the
+// first defect's norms (k_gate_init), so a request joins without a host
+// round trip.  The host replays the ring, at most kItemRing cycles behind.
+constexpr int kItemRing = 8;
+struct ItemGate {
+  int stop;                    // the stop rule held (or the first defect ended the solve)
+  int count;                   // cycles run in this solve
+  int maxcy, rule, fp64, report, pad0, pad1;
+  double bnorm, res0, tol, pad2;
+  double hist[kItemRing][6];   // per cycle: as CycleGate::hist
 };
 
 // Per-item caller arrays (reference-layout fields), by value.
@@ -280,14 +298,22 @@ inline unsigned gm_magic(unsigned d, int* sh) {
 // The end of a gated cycle, run by the cycle's last norm fold: g (nullptr:
 // no gate), the cycle's norms (out: 4 values; outc: the 2 relmax values of
 // an FP64 solve that reports them), and the instance's level descriptors.
+// With ig (per-instance gates, one per item of the fold): item t's norms
+// are out + t*4 and outc + t*2, its gate ig[Lt[item].gslot].
 struct GateArgs {
   CycleGate* g;
   const double* out;
   const double* outc;
   DevLevel* Lt;
   int nL, inst;
+  ItemGate* ig;
 };
 void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s);
+// Initialises the gates of the items of ll (FP64 finest levels) from the
+// first defect's norms out + t*4 = {max|r|, max|b|, ..}; an item whose
+// solve ends there (NaN, or the residual rule already holds) is stopped.
+void launch_gate_init(const LaunchList& ll, const double* out, ItemGate* ig, DevLevel* L, int nL,
+                      int maxcy, int rule, double tol, int report, hipStream_t s);
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum,
                      double* out, int ostride, hipStream_t s, const GateArgs* gate = nullptr);
 

@@ -1486,9 +1486,66 @@ __device__ void gate_check(const GateArgs& ga, const double* out) {
       if (ga.Lt[l].inst == ga.inst) ga.Lt[l].stop = 1;
 }
 
+// End of a gated cycle of one item of a batch (one thread): as gate_check,
+// with the instance's own gate and a ring of recent cycles.
+__device__ void item_gate_check(ItemGate* g, const double* out, const double* outc, DevLevel* Lt,
+                                int nL, int inst) {
+  if (g->stop) return;
+  const int n = g->count;
+  double* h = g->hist[n % kItemRing];
+  for (int q = 0; q < 4; ++q) h[q] = out[q];
+  h[4] = g->report ? outc[0] : 0.0;
+  h[5] = g->report ? outc[1] : 0.0;
+  const double rmax = out[0];
+  double relmax;
+  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
+  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
+  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
+  g->count = n + 1;
+  int stop = 0;
+  if (!isfinite(rmax)) {
+    stop = 1;
+  } else {
+    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
+    if (crit <= g->tol || n + 1 >= g->maxcy) stop = 1;
+  }
+  g->stop = stop;
+  if (stop)
+    for (int l = 0; l < nL; ++l)
+      if (Lt[l].inst == inst) Lt[l].stop = 1;
+}
+
+// One thread per item: the gate of a joining solve, from its first defect
+// (the decisions of the host's Context::prepare, with its operations).
+__global__ void k_gate_init(LaunchList ll, const double* __restrict__ out, ItemGate* ig,
+                            DevLevel* Lt, int nL, int maxcy, int rule, double tol, int report) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ll.nitems) return;
+  const DevLevel& L = Lt[ll.items[t]];
+  ItemGate* g = ig + L.gslot;
+  const double r = out[(size_t)t * 4], b = out[(size_t)t * 4 + 1];
+  const int fp64 = L.link < 0;
+  g->count = 0;
+  g->maxcy = maxcy;
+  g->rule = rule;
+  g->fp64 = fp64;
+  g->report = fp64 && report;
+  g->tol = tol;
+  g->bnorm = b;
+  g->res0 = b > 0.0 ? r / b : r;
+  int stop = 0;
+  if (!isfinite(r) || !isfinite(b)) stop = 1;
+  else if (rule == AMG_STOP_RESIDUAL && (b == 0.0 || g->res0 <= tol)) stop = 1;
+  g->stop = stop;
+  if (stop)
+    for (int l = 0; l < nL; ++l)
+      if (Lt[l].inst == L.inst) Lt[l].stop = 1;
+}
+
 // One block per item: thread t folds partials b0+t, b0+t+kFinBlock, ... in
 // order, then a fixed-shape tree combines the threads (deterministic).  With
-// a gate (a gated cycle's last fold, one item), thread 0 then ends the cycle.
+// a gate (a gated cycle's last fold, one item), thread 0 then ends the cycle;
+// with per-instance gates, thread 0 of every item's block ends its cycle.
 constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
                                                         int nvals, int op_sum, double* __restrict__ out,
@@ -1532,6 +1589,11 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
     __syncthreads();
   }
   if (ga.g && t == 0 && threadIdx.x == 0) gate_check(ga, ga.out);
+  if (ga.ig && threadIdx.x == 0) {
+    const DevLevel& L = ga.Lt[ll.items[t]];
+    item_gate_check(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, ga.Lt, ga.nL,
+                    L.inst);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2291,6 +2353,11 @@ void launch_zero(double* a, int n, hipStream_t s) {
 }
 void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s) {
   hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(64), 0, s, L, nL, inst);
+}
+void launch_gate_init(const LaunchList& ll, const double* out, ItemGate* ig, DevLevel* L, int nL,
+                      int maxcy, int rule, double tol, int report, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate_init, dim3(1), dim3(kMaxItems), 0, s, ll, out, ig, L, nL, maxcy, rule,
+                     tol, report);
 }
 void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int op_sum, double* out,
                      int ostride, hipStream_t s, const GateArgs* gate) {

@@ -2,14 +2,24 @@
 //
 // A worker thread owns one stream and advances every in-flight request by
 // one V-cycle per round, with one launch per level step for all of them
-// (Context::cycle_and_norms).  Requests that arrive while others run are
-// admitted at the next round boundary when a slot and HBM budget are free;
-// finished requests leave at once.  Each ensemble member keeps its
-// warm-start pressure resident, and its multigrid hierarchy stays bound to
-// a slot until the slot is needed by another member (least recently used).
-// Every per-request computation is independent of the rest of the batch
-// (per-item kernels, per-item reductions), so a result is bitwise the same
-// alone and in the mix.
+// (Context::cycle_and_norms); a round whose signature (requests and their
+// pressure buffers) came before runs as a captured graph.  Each request has
+// its own gate on the device (ItemGate): the round's last norm fold applies
+// the stop rule to every request, and a request that stops has its kernels
+// in later rounds return at once.  After a round, one copy brings every
+// gate into a pinned mirror, followed by an event; the host waits for that
+// event (not for the stream), replays the gates (Context::absorb_gate) and
+// retires the finished requests (null-space removal, ghosts and the copy
+// out, behind an event of their own).  A joining request's first defect
+// initialises its gate on the device, so joining takes no host round trip.
+// The code keeps kPipe rounds in flight; the stream is drained only when a
+// member's hierarchy must be bound or evicted (the descriptor tables
+// change), which happens once per member while the slots and the HBM budget
+// hold every member.  A member keeps its warm-start pressure resident, and
+// its hierarchy stays bound to a slot until the slot is needed by another
+// member (least recently used).  Every per-request computation is
+// independent of the rest of the batch (per-item kernels, per-item
+// reductions), so a result is bitwise the same alone and in the mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,8 +64,39 @@ struct Request {
 
 }  // namespace
 
+// Served rounds in flight (each with its own pinned copy of the gates).
+// One: a round ahead would keep the device busy through the host's turn
+// between rounds (tens of microseconds against rounds of milliseconds),
+// but a request would then join and leave one round later; two and three
+// rounds in flight were measured 5 and 10 % slower in solves/s at 1000
+// requests/s on the served mix, with longer latencies.
+constexpr int kPipe = 1;
+// Batched rounds captured as graphs (launch lists keyed by the round's
+// instances and pressure buffers), kept for the most recent signatures.
+constexpr int kRoundGraphs = 16;
+
 struct amg_server_s {
   Context ctx;
+  ItemGate* h_mirror = nullptr;      // kPipe x kMaxInstances, pinned
+  // round graphs: signature (precision, then instance id and pressure
+  // buffer per item) -> executable; a signature is captured when it comes
+  // back (signatures seen once are kept in 'seen')
+  struct RoundGraph {
+    std::vector<int> key;
+    hipGraphExec_t exec;
+    long last;
+  };
+  std::vector<RoundGraph> graphs;
+  std::deque<std::vector<int>> seen;
+  long tick = 0;
+  bool graphs_ok = true;
+  hipStream_t cap = nullptr;         // capture stream (never runs work)
+  void drop_round_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+    seen.clear();
+  }
+  int round(const std::vector<Instance*>& C);
   double budget = 0.0;
   size_t used = 0;
   int max_slots = 1, max_batch = 1;
@@ -100,14 +141,69 @@ int amg_server_s::bind(Member& m) {
       if (x.inst && !x.busy && (!victim || x.last_use < victim->last_use)) victim = &x;
     if (!victim) return AMG_EBUSY;
     used -= victim->inst->device_bytes();
+    drop_round_graphs();
     ctx.remove_instance(victim->inst);
     victim->inst = nullptr;
   }
   Instance* I = nullptr;
+  drop_round_graphs();                 // descriptor indices and tables change
   const int rc = ctx.add_instance(m.grid, &I);
   if (rc != AMG_OK) return rc;
   used += I->device_bytes();
   m.inst = I;
+  return AMG_OK;
+}
+
+// One batched gated cycle of the instances C (their cur set to the
+// parity of this cycle): from a captured graph when the signature has been
+// seen before, else launched directly.
+int amg_server_s::round(const std::vector<Instance*>& C) {
+  std::vector<int> key{C[0]->fp64 ? 1 : 0};
+  for (const Instance* I : C) {
+    key.push_back(I->id);
+    key.push_back(I->cur);
+  }
+  ++tick;
+  for (auto& g : graphs)
+    if (g.key == key) {
+      g.last = tick;
+      AMG_HIP(hipGraphLaunch(g.exec, ctx.stream));
+      return AMG_OK;
+    }
+  const bool again = std::find(seen.begin(), seen.end(), key) != seen.end();
+  if (!again || !graphs_ok) {
+    if (!again) {
+      seen.push_back(key);
+      if (seen.size() > 4 * kRoundGraphs) seen.pop_front();
+    }
+    return ctx.cycle_and_norms(C, kGateEach);
+  }
+  // capture on a stream of its own (no work of this one's is pending there)
+  hipStream_t run = ctx.stream;
+  ctx.stream = cap;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    ok = ctx.cycle_and_norms(C, kGateEach) == AMG_OK;
+    ok = hipStreamEndCapture(cap, &g) == hipSuccess && ok && g;
+  }
+  ctx.stream = run;
+  if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+  if (g) (void)hipGraphDestroy(g);
+  if (!ok) {
+    (void)hipGetLastError();
+    graphs_ok = false;
+    return ctx.cycle_and_norms(C, kGateEach);
+  }
+  if ((int)graphs.size() >= kRoundGraphs) {
+    auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                [](const RoundGraph& a, const RoundGraph& b) { return a.last < b.last; });
+    (void)hipGraphExecDestroy(lru->exec);
+    graphs.erase(lru);
+  }
+  graphs.push_back({key, ex, tick});
+  AMG_HIP(hipGraphLaunch(ex, ctx.stream));
   return AMG_OK;
 }
 
@@ -119,24 +215,129 @@ void amg_server_s::run() {
     const double* d_rhs;
     double* d_out;
     size_t nghost;
+    int spec;          // cycles enqueued for this solve
   };
+  struct Round {
+    hipEvent_t ev;
+    int mirror;
+    std::vector<int> rids;   // requests whose cycle the round ran
+  };
+  struct Leaving {
+    hipEvent_t ev;
+    std::vector<Slot> slots;
+    int rc;
+  };
+  std::map<int, Slot> live;       // requests on the device, by id
+  std::deque<Round> flight;
+  std::deque<Leaving> leaving;
+  std::vector<hipEvent_t> spare;
+  int next_mirror = 0;
+  int rc = AMG_OK;
+  auto event = [&]() {
+    hipEvent_t e = nullptr;
+    if (!spare.empty()) {
+      e = spare.back();
+      spare.pop_back();
+    } else if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  };
+  // requests whose copy-out completed (or every one after an error)
+  auto complete = [&](bool wait) {
+    while (!leaving.empty()) {
+      Leaving& l = leaving.front();
+      if (l.ev) {
+        const hipError_t q = wait ? hipEventSynchronize(l.ev) : hipEventQuery(l.ev);
+        if (q == hipErrorNotReady) return;
+        if (q != hipSuccess && l.rc == AMG_OK) l.rc = AMG_EHIP;
+        spare.push_back(l.ev);
+      }
+      const auto now = Clock::now();
+      {
+        std::lock_guard<std::mutex> g(mu);
+        for (const Slot& x : l.slots) {
+          Request& r = reqs[x.rid];
+          Member& m = members[r.member];
+          const Instance* I = x.inst;
+          r.iters = I->converged ? I->iters : ctx.cfg.maxcy + 1;
+          r.relres = I->relres;
+          r.status = l.rc != AMG_OK ? l.rc
+                     : (I->status != AMG_OK ? I->status : (I->converged ? AMG_OK : AMG_ENOTCONV));
+          r.t_done = now;
+          r.done = true;
+          m.busy = false;
+          m.last_use = now;
+          active.erase(std::find(active.begin(), active.end(), x.rid));
+        }
+        if (l.rc != AMG_OK) error = l.rc;
+      }
+      cv_done.notify_all();
+      leaving.pop_front();
+    }
+  };
+  // the oldest round: wait for it, replay its gates, retire what finished
+  auto land = [&]() {
+    Round rd = std::move(flight.front());
+    flight.pop_front();
+    if (rc == AMG_OK && hipEventSynchronize(rd.ev) != hipSuccess) rc = AMG_EHIP;
+    spare.push_back(rd.ev);
+    std::vector<Slot> D;
+    for (int rid : rd.rids) {
+      auto it = live.find(rid);
+      if (it == live.end()) continue;
+      Instance* I = it->second.inst;
+      if (rc == AMG_OK) rc = ctx.absorb_gate(I, h_mirror[(size_t)rd.mirror * kMaxInstances + I->gslot]);
+      if (I->done || rc != AMG_OK) {
+        D.push_back(it->second);
+        live.erase(it);
+      }
+    }
+    if (D.empty()) return;
+    Leaving l{nullptr, D, rc};
+    std::vector<Instance*> DI;
+    for (const Slot& x : D) DI.push_back(x.inst);
+    if (l.rc == AMG_OK) l.rc = ctx.finish(DI);
+    for (const Slot& x : D)
+      (void)hipMemcpyAsync(x.d_out, x.d_warm, x.nghost * 8, hipMemcpyDeviceToDevice, ctx.stream);
+    l.ev = event();
+    if (!l.ev || hipEventRecord(l.ev, ctx.stream) != hipSuccess) {
+      if (l.ev) spare.push_back(l.ev);
+      l.ev = nullptr;
+      (void)hipStreamSynchronize(ctx.stream);
+      if (l.rc == AMG_OK) l.rc = AMG_EHIP;
+    }
+    leaving.push_back(std::move(l));
+  };
+  auto drain = [&]() {
+    while (!flight.empty()) land();
+    complete(true);
+  };
+
   std::unique_lock<std::mutex> lk(mu);
   while (true) {
-    cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-    if (stop && active.empty()) break;
+    if (flight.empty() && leaving.empty())
+      cv_work.wait(lk, [&] { return stop || !queue.empty() || !live.empty(); });
+    if (stop && live.empty() && flight.empty() && leaving.empty()) break;
     // admission, in arrival order; a member runs one request at a time
-    // (its warm start is serial), so later requests of a busy member wait
+    // (its warm start is serial), so later requests of a busy member wait.
+    // Binding changes the descriptor tables: only with nothing in flight.
     std::vector<int> joining;
-    for (size_t q = 0; q < queue.size() && (int)active.size() + (int)joining.size() < max_batch;) {
+    bool need_drain = false;
+    for (size_t q = 0; q < queue.size() && (int)live.size() + (int)joining.size() < max_batch;) {
       const int rid = queue[q];
       Request& r = reqs[rid];
       Member& m = members[r.member];
       if (m.busy) { ++q; continue; }
-      const int rc = bind(m);
-      if (rc == AMG_EBUSY) break;
+      if (!m.inst && (!flight.empty() || !leaving.empty())) {
+        need_drain = true;
+        break;
+      }
+      const int brc = bind(m);
+      if (brc == AMG_EBUSY) break;
       queue.erase(queue.begin() + q);
-      if (rc != AMG_OK) {
-        r.status = rc;
+      if (brc != AMG_OK) {
+        r.status = brc;
         r.done = true;
         r.t_done = Clock::now();
         cv_done.notify_all();
@@ -145,67 +346,79 @@ void amg_server_s::run() {
       m.busy = true;
       joining.push_back(rid);
     }
-    // snapshot what the device work needs (members/reqs may grow meanwhile)
-    auto slot_of = [&](int rid) {
+    std::vector<Slot> J;
+    for (int rid : joining) {
       const Request& r = reqs[rid];
       const Member& m = members[r.member];
-      return Slot{rid, m.inst, m.d_warm, r.d_rhs, r.d_out, m.nghost};
-    };
-    std::vector<Slot> J, A;
-    for (int rid : joining) {
-      J.push_back(slot_of(rid));
+      J.push_back(Slot{rid, m.inst, m.d_warm, r.d_rhs, r.d_out, m.nghost, 0});
       active.push_back(rid);
     }
-    for (int rid : active) A.push_back(slot_of(rid));
+    int nslot = 0;
+    for (const Member& m : members)
+      if (m.inst) nslot = std::max(nslot, m.inst->gslot + 1);
     lk.unlock();
 
-    // join: RHS and warm start in, initial residual
-    std::vector<Instance*> S;
-    for (const Slot& x : J) {
-      x.inst->d_phi = x.d_warm;
-      x.inst->d_rhs = x.d_rhs;
-      S.push_back(x.inst);
+    // join: RHS and warm start in, first defect, gates
+    if (!J.empty() && rc == AMG_OK) {
+      std::vector<Instance*> S;
+      for (const Slot& x : J) {
+        x.inst->d_phi = x.d_warm;
+        x.inst->d_rhs = x.d_rhs;
+        S.push_back(x.inst);
+      }
+      rc = ctx.prepare_gated(S);
     }
-    int rc = S.empty() ? AMG_OK : ctx.prepare(S);
-    // one round: one V-cycle of every in-flight request, one launch per level step
-    std::vector<Instance*> C;
-    for (const Slot& x : A)
-      if (!x.inst->done) C.push_back(x.inst);
-    if (rc == AMG_OK && !C.empty()) {
-      rc = ctx.cycle_and_norms(C, false);
-      if (rc == AMG_OK && hipStreamSynchronize(ctx.stream) != hipSuccess) rc = AMG_EHIP;
-      if (rc == AMG_OK) ctx.post_norms(C);
+    for (const Slot& x : J) live[x.rid] = x;
+    // one round: one V-cycle of every live request (those that stopped on
+    // the device return at once), then the gates to the host
+    if (!live.empty() && rc == AMG_OK) {
+      Round rd{event(), next_mirror, {}};
+      next_mirror = (next_mirror + 1) % kPipe;
+      std::vector<Instance*> C;
+      std::vector<int> cur;
+      for (auto& [rid, x] : live) {
+        Instance* I = x.inst;
+        cur.push_back(I->cur);
+        if (!I->fp64) I->cur = x.spec & 1;   // the parity the device holds at this cycle
+        x.spec += 1;
+        C.push_back(I);
+        rd.rids.push_back(rid);
+      }
+      rc = round(C);
+      for (size_t q = 0; q < C.size(); ++q) C[q]->cur = cur[q];
+      if (rc == AMG_OK &&
+          hipMemcpyAsync(h_mirror + (size_t)rd.mirror * kMaxInstances, ctx.d_igate,
+                         sizeof(ItemGate) * nslot, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+        rc = AMG_EHIP;
+      if (rc == AMG_OK && (!rd.ev || hipEventRecord(rd.ev, ctx.stream) != hipSuccess)) rc = AMG_EHIP;
+      if (rc == AMG_OK) flight.push_back(std::move(rd));
+      else if (rd.ev) spare.push_back(rd.ev);
     }
-    // leave: finished requests write their solution (warm start kept resident)
-    std::vector<Instance*> D;
-    std::vector<Slot> DS;
-    for (const Slot& x : A)
-      if (x.inst->done || rc != AMG_OK) { D.push_back(x.inst); DS.push_back(x); }
-    if (!D.empty()) {
-      if (rc == AMG_OK) rc = ctx.finish(D);
-      for (const Slot& x : DS)
-        (void)hipMemcpyAsync(x.d_out, x.d_warm, x.nghost * 8, hipMemcpyDeviceToDevice, ctx.stream);
-      if (hipStreamSynchronize(ctx.stream) != hipSuccess && rc == AMG_OK) rc = AMG_EHIP;
+    if (rc != AMG_OK) {
+      // fail every live request
+      (void)hipStreamSynchronize(ctx.stream);
+      for (Round& r : flight) spare.push_back(r.ev);
+      flight.clear();
+      Leaving l{nullptr, {}, rc};
+      for (auto& [rid, x] : live) l.slots.push_back(x);
+      live.clear();
+      leaving.push_back(std::move(l));
     }
-    const auto now = Clock::now();
+    // keep kPipe - 1 rounds ahead of the host; with nothing left to enqueue,
+    // land the rest; a member to bind needs the stream drained
+    if (need_drain) {
+      drain();
+    } else {
+      while (!flight.empty() && ((int)flight.size() >= kPipe || live.empty())) {
+        // the copies out of retired requests precede the round in the stream
+        if (!leaving.empty()) complete(true);
+        land();
+      }
+      complete(live.empty() && flight.empty());
+    }
     lk.lock();
-    for (const Slot& x : DS) {
-      Request& r = reqs[x.rid];
-      Member& m = members[r.member];
-      Instance* I = x.inst;
-      r.iters = I->converged ? I->iters : ctx.cfg.maxcy + 1;
-      r.relres = I->relres;
-      r.status = rc != AMG_OK ? rc
-                 : (I->status != AMG_OK ? I->status : (I->converged ? AMG_OK : AMG_ENOTCONV));
-      r.t_done = now;
-      r.done = true;
-      m.busy = false;
-      m.last_use = now;
-      active.erase(std::find(active.begin(), active.end(), x.rid));
-    }
-    if (rc != AMG_OK) error = rc;
-    if (!DS.empty()) cv_done.notify_all();
   }
+  for (hipEvent_t e : spare) (void)hipEventDestroy(e);
 }
 
 extern "C" {
@@ -226,6 +439,20 @@ int amg_server_create(const amg_config* cfg, double hbm_budget_bytes, int max_sl
     delete s;
     return rc;
   }
+  if (hipHostMalloc((void**)&s->h_mirror, sizeof(ItemGate) * kMaxInstances * kPipe,
+                    hipHostMallocDefault) != hipSuccess) {
+    s->h_mirror = nullptr;
+    delete s;
+    set_error("amg_server_create: pinned allocation failed");
+    return AMG_ENOMEM;
+  }
+  if (hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking) != hipSuccess) {
+    s->cap = nullptr;
+    (void)hipHostFree(s->h_mirror);
+    delete s;
+    set_error("amg_server_create: stream creation failed");
+    return AMG_EHIP;
+  }
   s->budget = hbm_budget_bytes;
   s->max_slots = max_slots;
   s->max_batch = std::min(max_batch, kMaxItems);
@@ -245,6 +472,9 @@ int amg_server_destroy(amg_server s) {
   if (s->worker.joinable()) s->worker.join();
   for (auto& m : s->members)
     if (m.d_warm) (void)hipFree(m.d_warm);
+  if (s->h_mirror) (void)hipHostFree(s->h_mirror);
+  s->drop_round_graphs();
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
   return AMG_OK;
 }

@@ -112,6 +112,7 @@ int Context::init(const amg_config& c) {
   AMG_HIP(hipMalloc((void**)&d_out, sizeof(double) * kOutRegion * 4));
   AMG_HIP(hipHostMalloc((void**)&h_gate, sizeof(CycleGate), hipHostMallocDefault));
   AMG_HIP(hipMalloc((void**)&d_gate, sizeof(CycleGate)));
+  AMG_HIP(hipMalloc((void**)&d_igate, sizeof(ItemGate) * kMaxInstances));
   for (auto& e : ev) AMG_HIP(hipEventCreate(&e));
   return AMG_OK;
 }
@@ -137,6 +138,7 @@ Context::~Context() {
   if (h_out) (void)hipHostFree(h_out);
   if (d_gate) (void)hipFree(d_gate);
   if (h_gate) (void)hipHostFree(h_gate);
+  if (d_igate) (void)hipFree(d_igate);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -233,6 +235,18 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     if (rc != AMG_OK) return rc;
   }
   I->id = inst.empty() ? 0 : inst.back()->id + 1;
+  // the lowest gate slot no live instance holds
+  {
+    std::vector<char> used(kMaxInstances, 0);
+    for (const Instance* J : inst) used[J->gslot] = 1;
+    I->gslot = -1;
+    for (int q = 0; q < kMaxInstances && I->gslot < 0; ++q)
+      if (!used[q]) I->gslot = q;
+    if (I->gslot < 0) {
+      set_error("amg: too many instances on one context");
+      return AMG_ENOMEM;
+    }
+  }
   I->nx = g.nx; I->ny = g.ny; I->nz = g.nz;
   I->fp64 = (cfg.precision == AMG_FP64);
   const int L = (int)I->H.lv.size();
@@ -260,6 +274,7 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     d.py = I->H.bc[1][0] == AMG_BC_PERIODIC;
     d.pz = I->H.bc[2][0] == AMG_BC_PERIODIC;
     d.inst = I->id;
+    d.gslot = I->gslot;
     d.ncell = (int)H.ncell();
     d.link = -1;
     d.bcx0 = I->H.bc[0][0]; d.bcx1 = I->H.bc[0][1];
@@ -728,10 +743,13 @@ std::vector<Item> march_items(const Context& C, const std::vector<Instance*>& S,
 
 // Launches one cycle on every instance of S plus the norm kernels and the
 // device->host copy of the per-instance results (h_out, nvals per item);
-// gated (one instance): the gate check records the norms on the device
-// instead of the copy.
-int Context::cycle_and_norms(const std::vector<Instance*>& S, bool gated) {
+// gated: the gate checks record the norms on the device instead of the copy
+// (kGateOne: one instance, the context's gate; kGateEach: every instance its
+// own gate, ItemGate).
+int Context::cycle_and_norms(const std::vector<Instance*>& S, int gate_mode) {
   if (S.empty()) return AMG_OK;
+  const bool gated = gate_mode == kGateOne;
+  const bool each = gate_mode == kGateEach;
   const bool fp64 = S[0]->fp64;
   const bool relmax = fp64 && cfg.report_relmax;
   if (relmax) {
@@ -744,26 +762,34 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S, bool gated) {
   // gated (one instance): the cycle's last norm fold also ends the cycle
   const GateArgs ga{d_gate, d_out, d_out + kOutRegion * 2, dL, (int)hL.size(), S[0]->id};
   const GateArgs* gate_res = (gated && !relmax) ? &ga : nullptr;
+  // per-instance gates: item t of a list at offset ioff
+  auto each_args = [&](int ioff) {
+    return GateArgs{nullptr, d_out + (size_t)ioff * nv, d_out + kOutRegion * 2 + (size_t)ioff * 2,
+                    dL, (int)hL.size(), 0, d_igate};
+  };
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
+    const GateArgs ge = each_args(ioff);
+    const GateArgs* gr = (each && !relmax) ? &ge : gate_res;
     if (fp64) {
       launch_residual_max(dL, ll, part, stream);
-      launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream, gate_res);
+      launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream, gr);
     } else {
       launch_correct_defect32(dL, ll, part, stream);
-      launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream, gate_res);
+      launch_finalize(ll, part, 3, 0, d_out + (size_t)ioff * nv, nv, stream, gr);
     }
   });
   if (relmax) {
     auto citems = fine_items(*this, S, true, (long long)kBlock * kFinePer);
     for_lists(citems, [&](const LaunchList& ll, int ioff, int boff) {
       double* part = d_partial + (size_t)boff * nv;
+      const GateArgs ge = each_args(ioff);
       launch_change_max(dL, ll, part, stream);
       launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream,
-                      gated ? &ga : nullptr);
+                      gated ? &ga : each ? &ge : nullptr);
     });
   }
-  if (gated) return AMG_OK;
+  if (gated || each) return AMG_OK;
   (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
   if (relmax)
     (void)hipMemcpyAsync(h_out + kOutRegion * 2, d_out + kOutRegion * 2, sizeof(double) * S.size() * 2,
@@ -890,6 +916,82 @@ int Context::prepare(const std::vector<Instance*>& S) {
   return AMG_OK;
 }
 
+int Context::prepare_gated(const std::vector<Instance*>& S) {
+  const int nv = 4;
+  PtrList rhs{}, phi{};
+  std::vector<Item> items;
+  for (Instance* I : S) {
+    I->cur = 0;
+    I->iters = 0;
+    I->converged = false;
+    I->done = false;
+    I->status = AMG_OK;
+    I->gate_seen = false;
+    // the stop words of the instance's previous solve
+    launch_gate_reset(dL, (int)hL.size(), I->id, stream);
+    items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, (long long)kBlock * kWsumPer)});
+  }
+  double* c = d_out + kOutRegion * 3;
+  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
+    for (int t = 0; t < ll.nitems; ++t) {
+      rhs.p[t] = const_cast<double*>(S[ioff + t]->d_rhs);
+      phi.p[t] = S[ioff + t]->d_phi;
+    }
+    launch_ingest(dL, ll, rhs, phi, d_partial + boff, stream);
+    launch_finalize(ll, d_partial + boff, 1, 1, c + ioff, 1, stream);
+  });
+  const bool fp64 = S[0]->fp64;
+  const auto fitems = march_items(*this, S, false);
+  for_lists(fitems, [&](const LaunchList& ll, int ioff, int boff) {
+    double* part = d_partial + (size_t)boff * nv;
+    launch_first_defect(fp64, dL, ll, c + ioff, part, stream);
+    launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
+    launch_gate_init(ll, d_out + (size_t)ioff * nv, d_igate, dL, (int)hL.size(), cfg.maxcy,
+                     cfg.stop_rule, cfg.tol, cfg.report_relmax, stream);
+  });
+  AMG_HIP(hipGetLastError());
+  return AMG_OK;
+}
+
+int Context::absorb_gate(Instance* I, const ItemGate& g) {
+  if (!I->gate_seen) {
+    // the first defect (Context::prepare's decisions)
+    I->gate_seen = true;
+    I->bnorm = g.bnorm;
+    I->res0 = g.res0;
+    I->relres = g.res0;
+    I->relmax = 0.0;
+    if (!std::isfinite(g.bnorm) || !std::isfinite(g.res0)) {
+      I->status = AMG_ENAN;
+      I->done = true;
+    } else if (cfg.stop_rule == AMG_STOP_RESIDUAL && (g.bnorm == 0.0 || g.res0 <= cfg.tol)) {
+      I->converged = true;
+      I->done = true;
+    }
+  }
+  if (g.count < I->iters || g.count - I->iters > kItemRing) {
+    set_error("served round: the host lost track of the device's cycles");
+    return AMG_EHIP;
+  }
+  const std::vector<Instance*> A{I};
+  for (int c = I->iters; c < g.count; ++c) {
+    if (I->done) {
+      set_error("served round: the device ran past the host's stop");
+      return AMG_EHIP;
+    }
+    const double* h = g.hist[c % kItemRing];
+    for (int q = 0; q < 4; ++q) h_out[q] = h[q];
+    h_out[kOutRegion * 2] = h[4];
+    h_out[kOutRegion * 2 + 1] = h[5];
+    post_norms(A);
+  }
+  if ((g.stop != 0) != I->done) {
+    set_error("served round: the device and the host stop at different cycles");
+    return AMG_EHIP;
+  }
+  return AMG_OK;
+}
+
 int Context::finish(const std::vector<Instance*>& S) {
   std::vector<Item> cur;
   for (Instance* I : S) {
@@ -952,7 +1054,7 @@ int Context::gated_cycles(Instance* I, double* bytes) {
       bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
         for (int c = 0; c < kChainLen[q]; ++c) {
-          cycle_and_norms(A, true);
+          cycle_and_norms(A, kGateOne);
           if (!I->fp64) I->cur ^= 1;     // as post_norms does after a mixed cycle
         }
         ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
@@ -1032,7 +1134,7 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
         hipGraph_t g = nullptr;
         bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
-          cycle_and_norms(A, false);
+          cycle_and_norms(A, kGateNone);
           ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
         }
         if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
@@ -1044,9 +1146,9 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
         }
       }
       if (gexec[gi]) AMG_HIP(hipGraphLaunch(gexec[gi], stream));
-      else cycle_and_norms(A, false);
+      else cycle_and_norms(A, kGateNone);
     } else {
-      cycle_and_norms(A, false);
+      cycle_and_norms(A, kGateNone);
     }
     AMG_HIP(hipStreamSynchronize(stream));
     post_norms(A);

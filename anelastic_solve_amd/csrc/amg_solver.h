@@ -17,6 +17,8 @@ void set_error(const std::string& msg);
 const std::string& last_error();
 
 constexpr int kMaxInstances = 1024;
+// gate modes of Context::cycle_and_norms
+enum { kGateNone = 0, kGateOne = 1, kGateEach = 2 };
 
 #define AMG_HIP(call)                                                        \
   do {                                                                       \
@@ -45,6 +47,7 @@ struct DevArena {
 // One Poisson problem resident on the device.
 struct Instance {
   int id = 0;
+  int gslot = 0;                 // slot of its gate in Context::d_igate
   Hierarchy H;
   int nx = 0, ny = 0, nz = 0;
   bool fp64 = false;             // V-cycle precision (true: plain FP64)
@@ -72,6 +75,7 @@ struct Instance {
   int last_iters = 0;            // cycles of the previous solve (the gated batch size)
   double bnorm = 0.0, relres = 0.0, relmax = 0.0, res0 = 0.0;
   int status = AMG_OK;
+  bool gate_seen = false;        // served: the host has read the gate of the first defect
   size_t bytes_per_cycle = 0;
   double* d_phi = nullptr;       // caller arrays of the current solve
   const double* d_rhs = nullptr;
@@ -97,6 +101,7 @@ struct Context {
   double* h_out = nullptr;       // pinned mirror
   CycleGate* d_gate = nullptr;   // gated cycles: device control block
   CycleGate* h_gate = nullptr;   // and its pinned mirror
+  ItemGate* d_igate = nullptr;   // per-instance gates (served rounds), kMaxInstances
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // single-instance graphs: [0] fp64 cycle, [1]/[2] mixed cycle for cur = 0/1
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
@@ -120,7 +125,14 @@ struct Context {
   int prepare(const std::vector<Instance*>& S);
   const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
-  int cycle_and_norms(const std::vector<Instance*>& S, bool gated);
+  int cycle_and_norms(const std::vector<Instance*>& S, int gate_mode);
+  // the start of a solve for a served round: like prepare, with the gates
+  // of S initialised on the device instead of a host round trip
+  int prepare_gated(const std::vector<Instance*>& S);
+  // replays the cycles that instance I ran on the device since the host's
+  // last look (its gate as copied after a round); AMG_EHIP when the device
+  // and the host disagree
+  int absorb_gate(Instance* I, const ItemGate& g);
   int gated_cycles(Instance* I, double* bytes);
   void post_norms(const std::vector<Instance*>& S);
   int run_kernel(Instance* I, int which, int level, int reps, double* bytes);

@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances section")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream section")
     ap.add_argument("--no-shear", action="store_true", help="skip the shear-layer sequence")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 1000.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200.0, 500.0, 1000.0, 2000.0])
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--shear-steps", type=int, default=20)
     ap.add_argument("--no-production", action="store_true",
