@@ -50,6 +50,7 @@ class amg_config(C.Structure):
         ("strict", C.c_int),
         ("use_graph", C.c_int),
         ("coarse_direct", C.c_int),
+        ("pair_cycles", C.c_int),
     ]
 
 

@@ -169,6 +169,7 @@ void amg_config_default(amg_config* c) {
   c->report_relmax = 0;
   c->use_graph = 1;
   c->coarse_direct = 1;
+  c->pair_cycles = 1;
 }
 
 // mgMK/mg_read_params.F90:44-214
@@ -203,7 +204,7 @@ int amg_read_params(const char* infile, amg_config* cfg) {
     return true;
   };
   if (!logical("Strict", c.strict) || !logical("Verbose", c.verbose) ||
-      !logical("CoarseDirect", c.coarse_direct))
+      !logical("CoarseDirect", c.coarse_direct) || !logical("PairCycles", c.pair_cycles))
     return AMG_EPARSE;
   const char* lk[3] = {"nxlevels", "nylevels", "nzlevels"};
   for (int d = 0; d < 3; ++d) {

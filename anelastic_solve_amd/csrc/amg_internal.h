@@ -30,7 +30,14 @@ struct DevLevel {
   int ncell;                 // nx*ny*nz
   int link;                  // FP64 finest entry: index of the FP32 finest level (mixed)
   int nullsp;                // the operator is singular (periodic/Neumann): constants in the null space
-  int stop;                  // gated cycles: set once the stop rule held; the kernels then return
+  // the instance's solve state, written by the device (the gate at the end
+  // of every cycle) on every level of the instance:
+  int stop;                  // the stop rule held: the instance's kernels return at once
+  int warm;                  // this cycle continues the previous one's correction (the second
+                             //   of a pair): the finest level's first sweep reads the iterate
+  int skipc;                 // this cycle is the first of a pair: no FP64 correction after it
+  int pcur;                  // FP64 pressure buffers of a mixed solve: 1 when u2 holds the current
+  int pairok;                // the instance may pair cycles (mixed, residual stop rule)
   int gslot;                 // the instance's slot in the per-instance gate array
   unsigned mhx, mny;         // ceil(2^32/hx), ceil(2^32/ny) (0 for a divisor of 1): e/hx = umulhi(e, mhx)
                              // exactly while e*hx < 2^32 (the coarse tail's small levels)
@@ -68,38 +75,49 @@ struct LaunchList {
   int prefix[kMaxItems + 1];   // block prefix
 };
 
-// Gated cycles of one instance: the cycles of a batch are enqueued without a
-// host round trip between them.  The cycle's last norm fold ends every
-// cycle: it records the cycle's norms in hist and applies the stop rule of
-// the host's norm step to them; once it stops, it sets DevLevel::stop of
-// the instance's levels and every later kernel of the batch returns at
-// once.  The host then replays hist through its own norm step.
-constexpr int kGateHist = 64;
-struct CycleGate {
-  int stop;                    // set when the stop rule holds (or hist is full)
-  int count;                   // cycles run in this batch
-  int iters0;                  // cycles before the batch
-  int maxcy, rule, fp64, report, pad;
-  double bnorm, tol;
-  double hist[kGateHist][6];   // per cycle: the 4 norms of the cycle, and the 2 of relmax (FP64)
-};
-
-// Gated cycles of a batch of instances (served rounds): one gate per
-// instance, in one array indexed by DevLevel::gslot, so that the host reads
-// every gate of a round with one copy.  Each cycle's last norm fold records
-// the cycle's norms in the instance's ring (entry count % kItemRing) and
-// applies the stop rule of the host's norm step; a stop sets DevLevel::stop
-// of the instance's levels.  The gate is initialised on the device from the
-// first defect's norms (k_gate_init), so a request joins without a host
-// round trip.  The host replays the ring, at most kItemRing cycles behind.
-constexpr int kItemRing = 8;
+// The gate of a solve (one per instance, in one array indexed by
+// DevLevel::gslot, so that the host reads every gate of a batch with one
+// copy).  The host enqueues cycles without waiting for their norms: the last
+// norm fold of every cycle runs the gate, which records the norms of each
+// FP64 correction in a ring (entry count % kItemRing), applies the stop
+// rule of the host's norm step to them and writes the instance's state
+// words (DevLevel::stop, warm, skipc, pcur); once it stops, every later
+// kernel of the instance returns at once.  The host replays the ring
+// (Context::absorb_gate), at most kItemRing corrections behind.
+//
+// Paired cycles (mixed precision, residual stop rule): after a correction
+// whose residual r (and r' at the previous one, m V-cycles earlier)
+// predicts that one more cycle cannot reach the tolerance, r * rho > tol
+// with rho = (r / r')^(1/m), the next two V-cycles share one correction:
+// the first skips it, the second continues from the first's iterate.  The
+// per-cycle rate of the FP32 V-cycle is unchanged by it, so the stop falls
+// at the same cycle with fewer FP64 passes.  The rate is estimated from the
+// solve's own history only, so a solve gives the same bits whatever ran
+// before it or beside it.
+constexpr int kItemRing = 16;
+// zero-guess bits of a smoothing launch: the instance's state word decides
+// whether the sweep starts from zero (kZgDyn) or continues (kZgWarm)
+constexpr int kZgDyn = 4;
+constexpr int kZgWarm = 8;
+constexpr double kPairRho0 = 1e-3;   // the first correction: pair when res0 * kPairRho0 > tol
 struct ItemGate {
   int stop;                    // the stop rule held (or the first defect ended the solve)
-  int count;                   // cycles run in this solve
-  int maxcy, rule, fp64, report, pad0, pad1;
-  double bnorm, res0, tol, pad2;
-  double hist[kItemRing][6];   // per cycle: as CycleGate::hist
+  int count;                   // corrections recorded in this solve
+  int cycles;                  // V-cycles run in this solve
+  int maxcy, rule, fp64, report, pair;
+  double bnorm, res0, tol;
+  double rprev;                // relative residual at the last correction (res0 before the first)
+  double hist[kItemRing][6];   // per correction: the 4 norms of its fold, the 2 of relmax (FP64)
+  int hcyc[kItemRing];         // per correction: the V-cycles it closes (1 or 2)
 };
+
+// Whether the next correction should close two V-cycles (the rule above;
+// multiplications only, so the host's replay decides alike).
+__host__ __device__ inline bool pair_next(double r, double rp, int m, double tol) {
+  if (!(rp > 0.0)) return false;
+  const double q = r / rp;
+  return m == 1 ? r * q > tol : (r * r) * q > tol * tol;
+}
 
 // Per-item caller arrays (reference-layout fields), by value.
 struct PtrList {
@@ -295,20 +313,18 @@ inline unsigned gm_magic(unsigned d, int* sh) {
   *sh = s1 | (s2 << 8);
   return (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
 }
-// The end of a gated cycle, run by the cycle's last norm fold: g (nullptr:
-// no gate), the cycle's norms (out: 4 values; outc: the 2 relmax values of
-// an FP64 solve that reports them), and the instance's level descriptors.
-// With ig (per-instance gates, one per item of the fold): item t's norms
-// are out + t*4 and outc + t*2, its gate ig[Lt[item].gslot].
+// The end of a gated cycle, run by the cycle's last norm fold (ig nullptr:
+// no gate): item t's norms are out + t*4 (and outc + t*2, the 2 relmax
+// values of an FP64 solve that reports them), its gate ig[Lt[item].gslot].
 struct GateArgs {
-  CycleGate* g;
   const double* out;
   const double* outc;
   DevLevel* Lt;
-  int nL, inst;
+  int nL;
   ItemGate* ig;
 };
-void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s);
+// Clears the state words of the instances of the items of ll.
+void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s);
 // Initialises the gates of the items of ll (FP64 finest levels) from the
 // first defect's norms out + t*4 = {max|r|, max|b|, ..}; an item whose
 // solve ends there (NaN, or the residual rule already holds) is stopped.

@@ -44,6 +44,17 @@ __device__ __forceinline__ int find_item(const int* prefix, int n, int b) {
 // A gated cycle has stopped (DevLevel::stop): the kernel returns at once.
 __device__ __forceinline__ bool level_off(const DevLevel& L) { return L.stop != 0; }
 
+// Zero-guess flags of a smoothing launch (bit 0: the other colour is zero;
+// bit 1: the own colour's old values are zero).  With kZgDyn, the finest
+// level's first sweep of a mixed cycle, they hold only when the cycle does
+// not continue the previous one's iterate (DevLevel::warm); kZgWarm marks
+// the launch that does that sweep when the cycle continues it.
+__device__ __forceinline__ int zero_flags(const DevLevel& L, int zero_guess) {
+  if (zero_guess & kZgWarm) return 0;
+  if (!(zero_guess & kZgDyn)) return zero_guess;
+  return L.warm ? 0 : (zero_guess & 3);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);
 // remap so that every XCD walks a contiguous range of tiles and neighbouring
 // rows share an L2.  A bijection on [0, n).
@@ -240,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.hx * L.ny * L.nz)
     rb_point_elem<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), e, parity,
-                     zero_guess);
+                     zero_flags(L, zero_guess));
 }
 
 // ---------------------------------------------------------------------------
@@ -541,10 +552,15 @@ __global__ __launch_bounds__(1024, (LEAN && sizeof(T) == 4) ? 8 : 4) void k_zlin
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (level_off(L)) return;
+  // the first sweep of a mixed cycle's finest level runs as two launches,
+  // from zero (NBR false) and continuing the iterate (kZgWarm): one returns
+  if (!NBR && (zero_guess & kZgDyn) && L.warm) return;
+  if ((zero_guess & kZgWarm) && !L.warm) return;
   // threadIdx.y is constant across a wave (blockDim.x == 64): say so to the compiler
   const int seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  zline_group<T, S, C, NBR, LEAN>(L, parity, zero_guess, (b - ll.prefix[t]) * 64 + threadIdx.x, seg,
-                            threadIdx.x, blockDim.y, reinterpret_cast<T*>(lds_raw), true);
+  zline_group<T, S, C, NBR, LEAN>(L, parity, zero_flags(L, zero_guess),
+                                  (b - ll.prefix[t]) * 64 + threadIdx.x, seg, threadIdx.x, blockDim.y,
+                                  reinterpret_cast<T*>(lds_raw), true);
 }
 
 template <class T> struct TailNz;
@@ -664,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_zline_col(const DevLevel* __restrict
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < L.hx * L.ny)
     zline_column_any<T>(L, gfield<T>(L.u), gfield<const T>((const void*)L.f), coefs<T>(L), parity,
-                        zero_guess, e);
+                        zero_flags(L, zero_guess), e);
 }
 
 // ---------------------------------------------------------------------------
@@ -1420,8 +1436,14 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
   if (level_off(L)) return;
-  const DevLevel& S = Lt[L.link];
+  // the first cycle of a pair: no correction (its gate reads no norms)
+  if (MODE == kCorrDefect32 && L.skipc) return;
+  // the FP32 finest level (mixed); an FP64 solve has none and reads no correction
+  const DevLevel& S = Lt[L.link >= 0 ? L.link : ll.items[t]];
   const int hx = L.hx, nx = L.nx, ny = L.ny, nz = L.nz;
+  // the current FP64 pressure and its ping-pong partner
+  const void* pin = L.pcur ? L.u2 : L.u;
+  void* pout = L.pcur ? L.u : L.u2;
   const int plane = 2 * hx * ny;
   const int ncb = (plane + kBlock - 1) / kBlock;
   const int lb = b - ll.prefix[t];
@@ -1441,9 +1463,9 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
     const auto cy = co + 3 * nx;
     const auto cz = cy + 3 * ny;
     const double cs = (SHIFT && L.nullsp) ? csum[t] / L.vsum : 0.0;
-    fine_march<MODE, NV, SHIFT>(gfield<const double>((const void*)L.u),
+    fine_march<MODE, NV, SHIFT>(gfield<const double>(pin),
                          gfield<const float>((const void*)S.u), gfield<double>(L.f), cs,
-                         gfield<double>(L.u2),
+                         gfield<double>(pout),
                          gfield<float>(S.f), cz, nz, plane, col, (j * 2 + oth) * hx + ml,
                          (j * 2 + oth) * hx + mr, (jm * 2 + oth) * hx + m, (jp * 2 + oth) * hx + m,
                          co[i], co[nx + i], cy[j], cy[ny + j], cy[2 * ny + j], co[2 * nx + i], k0, k1, v);
@@ -1451,51 +1473,41 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
   block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
 }
 
-// End of a gated cycle (one thread): record the cycle's norms and apply the
-// stop rule of the host's norm step (Context::post_norms) to them, with the
-// same double operations, so the host's replay reaches the same decision.
-// On a stop it sets the stop word of every level descriptor of instance
-// inst: the kernels of the later cycles of the batch read it with the rest
-// of their descriptor and return at once.
-__device__ void gate_check(const GateArgs& ga, const double* out) {
-  CycleGate* g = ga.g;
-  if (g->stop) return;
-  const int n = g->count;
-  double* h = g->hist[n];
-  for (int q = 0; q < 4; ++q) h[q] = out[q];
-  h[4] = g->report ? ga.outc[0] : 0.0;
-  h[5] = g->report ? ga.outc[1] : 0.0;
-  const double rmax = out[0];
-  double relmax;
-  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
-  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
-  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
-  g->count = n + 1;
-  const int iters = g->iters0 + n + 1;
-  int stop = 0;
-  if (!isfinite(rmax)) {
-    stop = 1;
-  } else {
-    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
-    if (crit <= g->tol || iters >= g->maxcy) stop = 1;
+// Writes the state words of instance inst on all its levels (a negative
+// value leaves that word as it is; pflip toggles pcur).
+__device__ void set_words(DevLevel* Lt, int nL, int inst, int stop, int warm, int skipc, int pflip) {
+  for (int l = 0; l < nL; ++l) {
+    DevLevel& d = Lt[l];
+    if (d.inst != inst) continue;
+    if (stop >= 0) d.stop = stop;
+    if (warm >= 0) d.warm = warm;
+    if (skipc >= 0) d.skipc = skipc;
+    if (pflip) d.pcur ^= 1;
   }
-  if (n + 1 >= kGateHist) stop = 1;   // history full: the host runs the next batch
-  g->stop = stop;
-  if (stop)
-    for (int l = 0; l < ga.nL; ++l)
-      if (ga.Lt[l].inst == ga.inst) ga.Lt[l].stop = 1;
 }
 
-// End of a gated cycle of one item of a batch (one thread): as gate_check,
-// with the instance's own gate and a ring of recent cycles.
+// End of a gated cycle of one instance (one thread; L its item's level,
+// the FP64 finest): the first cycle of a pair only hands over to the
+// second; otherwise the correction's norms are recorded and the stop rule
+// of the host's norm step (Context::post_norms) applied to them with the
+// same double operations, so the host's replay reaches the same decision;
+// then the next cycle is planned (paired or not).
 __device__ void item_gate_check(ItemGate* g, const double* out, const double* outc, DevLevel* Lt,
-                                int nL, int inst) {
+                                int nL, const DevLevel& L) {
   if (g->stop) return;
+  const int inst = L.inst;
+  g->cycles += 1;
+  if (L.skipc) {
+    set_words(Lt, nL, inst, -1, 1, 0, 0);
+    return;
+  }
+  const int m = L.warm ? 2 : 1;
   const int n = g->count;
   double* h = g->hist[n % kItemRing];
   for (int q = 0; q < 4; ++q) h[q] = out[q];
   h[4] = g->report ? outc[0] : 0.0;
   h[5] = g->report ? outc[1] : 0.0;
+  g->hcyc[n % kItemRing] = m;
   const double rmax = out[0];
   double relmax;
   if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
@@ -1507,16 +1519,18 @@ __device__ void item_gate_check(ItemGate* g, const double* out, const double* ou
     stop = 1;
   } else {
     const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
-    if (crit <= g->tol || n + 1 >= g->maxcy) stop = 1;
+    if (crit <= g->tol || g->cycles >= g->maxcy) stop = 1;
   }
+  const bool pair = !stop && g->pair && g->cycles + 2 <= g->maxcy &&
+                    pair_next(relres, g->rprev, m, g->tol);
+  g->rprev = relres;
   g->stop = stop;
-  if (stop)
-    for (int l = 0; l < nL; ++l)
-      if (Lt[l].inst == inst) Lt[l].stop = 1;
+  set_words(Lt, nL, inst, stop, 0, pair ? 1 : 0, g->fp64 ? 0 : 1);
 }
 
-// One thread per item: the gate of a joining solve, from its first defect
-// (the decisions of the host's Context::prepare, with its operations).
+// One thread per item: the gate of a starting solve, from its first defect
+// (the decisions of the host's Context::prepare, with its operations), and
+// the plan of its first cycle.
 __global__ void k_gate_init(LaunchList ll, const double* __restrict__ out, ItemGate* ig,
                             DevLevel* Lt, int nL, int maxcy, int rule, double tol, int report) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1526,20 +1540,22 @@ __global__ void k_gate_init(LaunchList ll, const double* __restrict__ out, ItemG
   const double r = out[(size_t)t * 4], b = out[(size_t)t * 4 + 1];
   const int fp64 = L.link < 0;
   g->count = 0;
+  g->cycles = 0;
   g->maxcy = maxcy;
   g->rule = rule;
   g->fp64 = fp64;
   g->report = fp64 && report;
+  g->pair = L.pairok;
   g->tol = tol;
   g->bnorm = b;
   g->res0 = b > 0.0 ? r / b : r;
+  g->rprev = g->res0;
   int stop = 0;
   if (!isfinite(r) || !isfinite(b)) stop = 1;
   else if (rule == AMG_STOP_RESIDUAL && (b == 0.0 || g->res0 <= tol)) stop = 1;
   g->stop = stop;
-  if (stop)
-    for (int l = 0; l < nL; ++l)
-      if (Lt[l].inst == L.inst) Lt[l].stop = 1;
+  const bool pair = !stop && g->pair && maxcy >= 2 && g->res0 * kPairRho0 > tol;
+  set_words(Lt, nL, L.inst, stop, 0, pair ? 1 : 0, 0);
 }
 
 // One block per item: thread t folds partials b0+t, b0+t+kFinBlock, ... in
@@ -1588,11 +1604,10 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
     }
     __syncthreads();
   }
-  if (ga.g && t == 0 && threadIdx.x == 0) gate_check(ga, ga.out);
   if (ga.ig && threadIdx.x == 0) {
     const DevLevel& L = ga.Lt[ll.items[t]];
     item_gate_check(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, ga.Lt, ga.nL,
-                    L.inst);
+                    L);
   }
 }
 
@@ -2144,10 +2159,18 @@ inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) /
 
 }  // namespace
 
-// Clears the stop words of instance inst after a batch.
-__global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, int inst) {
-  for (int l = threadIdx.x; l < nL; l += blockDim.x)
-    if (Lt[l].inst == inst) Lt[l].stop = 0;
+// Clears the state words of the instances of the items of ll.
+__global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, LaunchList ll) {
+  for (int l = threadIdx.x; l < nL; l += blockDim.x) {
+    bool mine = false;
+    for (int t = 0; t < ll.nitems; ++t) mine = mine || Lt[ll.items[t]].inst == Lt[l].inst;
+    if (mine) {
+      Lt[l].stop = 0;
+      Lt[l].warm = 0;
+      Lt[l].skipc = 0;
+      Lt[l].pcur = 0;
+    }
+  }
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -2172,15 +2195,14 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
   if (fp64) LAUNCH(k_rb_point<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
   else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
 }
-void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
-                  int segs, hipStream_t s, int form) {
+static void zline_variant(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                          int zero_guess, int segs, hipStream_t s, int form, bool nbr) {
   const dim3 blk(64, segs);
   // reduced-system state (two buffers in FP32, one in FP64) + staged rows
   const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
 #define ZL(T, CH, NBR, LEAN)                                                                      \
   hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR, LEAN>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, \
                      zero_guess)
-  const bool nbr = !(zero_guess & 1);
   // FP64 always takes the lean form (the short one does not fit its
   // registers).  FP32 takes it when there are more blocks than CUs, so that
   // blocks can pair up; a smaller launch is latency-bound and takes the
@@ -2196,6 +2218,18 @@ void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity
     else ZL(float, kZChunk32, false, false);
   }
 #undef ZL
+}
+void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
+                  int segs, hipStream_t s, int form) {
+  // kZgDyn with a zero other colour: the sweep from zero, then the ordinary
+  // sweep for the instances whose cycle continues the iterate; each kernel
+  // returns at once for the instances of the other
+  if ((zero_guess & kZgDyn) && (zero_guess & 1)) {
+    zline_variant(fp64, L, ll, parity, zero_guess, segs, s, form, false);
+    zline_variant(fp64, L, ll, parity, kZgWarm, segs, s, form, true);
+    return;
+  }
+  zline_variant(fp64, L, ll, parity, zero_guess, segs, s, form, !(zero_guess & 1));
 }
 void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                       int zero_guess, hipStream_t s) {
@@ -2351,8 +2385,8 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
 }
-void launch_gate_reset(DevLevel* L, int nL, int inst, hipStream_t s) {
-  hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(64), 0, s, L, nL, inst);
+void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(256), 0, s, L, nL, ll);
 }
 void launch_gate_init(const LaunchList& ll, const double* out, ItemGate* ig, DevLevel* L, int nL,
                       int maxcy, int rule, double tol, int report, hipStream_t s) {

@@ -2,8 +2,8 @@
 //
 // A worker thread owns one stream and advances every in-flight request by
 // one V-cycle per round, with one launch per level step for all of them
-// (Context::cycle_and_norms); a round whose signature (requests and their
-// pressure buffers) came before runs as a captured graph.  Each request has
+// (Context::cycle_and_norms); a round whose signature (its requests) came
+// before runs as a captured graph.  Each request has
 // its own gate on the device (ItemGate): the round's last norm fold applies
 // the stop rule to every request, and a request that stops has its kernels
 // in later rounds return at once.  After a round, one copy brings every
@@ -72,15 +72,15 @@ struct Request {
 // requests/s on the served mix, with longer latencies.
 constexpr int kPipe = 1;
 // Batched rounds captured as graphs (launch lists keyed by the round's
-// instances and pressure buffers), kept for the most recent signatures.
+// instances), kept for the most recent signatures.
 constexpr int kRoundGraphs = 16;
 
 struct amg_server_s {
   Context ctx;
   ItemGate* h_mirror = nullptr;      // kPipe x kMaxInstances, pinned
-  // round graphs: signature (precision, then instance id and pressure
-  // buffer per item) -> executable; a signature is captured when it comes
-  // back (signatures seen once are kept in 'seen')
+  // round graphs: signature (precision, then the instance ids) ->
+  // executable; a signature is captured when it comes back (signatures
+  // seen once are kept in 'seen')
   struct RoundGraph {
     std::vector<int> key;
     hipGraphExec_t exec;
@@ -154,14 +154,12 @@ int amg_server_s::bind(Member& m) {
   return AMG_OK;
 }
 
-// One batched gated cycle of the instances C (their cur set to the
-// parity of this cycle): from a captured graph when the signature has been
-// seen before, else launched directly.
+// One batched gated cycle of the instances C: from a captured graph when
+// the signature has been seen before, else launched directly.
 int amg_server_s::round(const std::vector<Instance*>& C) {
   std::vector<int> key{C[0]->fp64 ? 1 : 0};
   for (const Instance* I : C) {
     key.push_back(I->id);
-    key.push_back(I->cur);
   }
   ++tick;
   for (auto& g : graphs)
@@ -176,7 +174,7 @@ int amg_server_s::round(const std::vector<Instance*>& C) {
       seen.push_back(key);
       if (seen.size() > 4 * kRoundGraphs) seen.pop_front();
     }
-    return ctx.cycle_and_norms(C, kGateEach);
+    return ctx.cycle_and_norms(C);
   }
   // capture on a stream of its own (no work of this one's is pending there)
   hipStream_t run = ctx.stream;
@@ -185,7 +183,7 @@ int amg_server_s::round(const std::vector<Instance*>& C) {
   hipGraphExec_t ex = nullptr;
   bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
   if (ok) {
-    ok = ctx.cycle_and_norms(C, kGateEach) == AMG_OK;
+    ok = ctx.cycle_and_norms(C) == AMG_OK;
     ok = hipStreamEndCapture(cap, &g) == hipSuccess && ok && g;
   }
   ctx.stream = run;
@@ -194,7 +192,7 @@ int amg_server_s::round(const std::vector<Instance*>& C) {
   if (!ok) {
     (void)hipGetLastError();
     graphs_ok = false;
-    return ctx.cycle_and_norms(C, kGateEach);
+    return ctx.cycle_and_norms(C);
   }
   if ((int)graphs.size() >= kRoundGraphs) {
     auto lru = std::min_element(graphs.begin(), graphs.end(),
@@ -215,7 +213,6 @@ void amg_server_s::run() {
     const double* d_rhs;
     double* d_out;
     size_t nghost;
-    int spec;          // cycles enqueued for this solve
   };
   struct Round {
     hipEvent_t ev;
@@ -287,7 +284,8 @@ void amg_server_s::run() {
       auto it = live.find(rid);
       if (it == live.end()) continue;
       Instance* I = it->second.inst;
-      if (rc == AMG_OK) rc = ctx.absorb_gate(I, h_mirror[(size_t)rd.mirror * kMaxInstances + I->gslot]);
+      if (rc == AMG_OK)
+        rc = ctx.absorb_gate(I, h_mirror[(size_t)rd.mirror * kMaxInstances + I->gslot], nullptr);
       if (I->done || rc != AMG_OK) {
         D.push_back(it->second);
         live.erase(it);
@@ -350,7 +348,7 @@ void amg_server_s::run() {
     for (int rid : joining) {
       const Request& r = reqs[rid];
       const Member& m = members[r.member];
-      J.push_back(Slot{rid, m.inst, m.d_warm, r.d_rhs, r.d_out, m.nghost, 0});
+      J.push_back(Slot{rid, m.inst, m.d_warm, r.d_rhs, r.d_out, m.nghost});
       active.push_back(rid);
     }
     int nslot = 0;
@@ -366,7 +364,7 @@ void amg_server_s::run() {
         x.inst->d_rhs = x.d_rhs;
         S.push_back(x.inst);
       }
-      rc = ctx.prepare_gated(S);
+      rc = ctx.prepare_async(S);
     }
     for (const Slot& x : J) live[x.rid] = x;
     // one round: one V-cycle of every live request (those that stopped on
@@ -375,17 +373,11 @@ void amg_server_s::run() {
       Round rd{event(), next_mirror, {}};
       next_mirror = (next_mirror + 1) % kPipe;
       std::vector<Instance*> C;
-      std::vector<int> cur;
       for (auto& [rid, x] : live) {
-        Instance* I = x.inst;
-        cur.push_back(I->cur);
-        if (!I->fp64) I->cur = x.spec & 1;   // the parity the device holds at this cycle
-        x.spec += 1;
-        C.push_back(I);
+        C.push_back(x.inst);
         rd.rids.push_back(rid);
       }
       rc = round(C);
-      for (size_t q = 0; q < C.size(); ++q) C[q]->cur = cur[q];
       if (rc == AMG_OK &&
           hipMemcpyAsync(h_mirror + (size_t)rd.mirror * kMaxInstances, ctx.d_igate,
                          sizeof(ItemGate) * nslot, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)

@@ -110,19 +110,15 @@ int Context::init(const amg_config& c) {
   AMG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   AMG_HIP(hipHostMalloc((void**)&h_out, sizeof(double) * kOutRegion * 4, hipHostMallocDefault));
   AMG_HIP(hipMalloc((void**)&d_out, sizeof(double) * kOutRegion * 4));
-  AMG_HIP(hipHostMalloc((void**)&h_gate, sizeof(CycleGate), hipHostMallocDefault));
-  AMG_HIP(hipMalloc((void**)&d_gate, sizeof(CycleGate)));
   AMG_HIP(hipMalloc((void**)&d_igate, sizeof(ItemGate) * kMaxInstances));
+  AMG_HIP(hipHostMalloc((void**)&h_igate, sizeof(ItemGate) * kMaxInstances, hipHostMallocDefault));
   for (auto& e : ev) AMG_HIP(hipEventCreate(&e));
   return AMG_OK;
 }
 
 void Context::drop_graphs() {
-  for (auto& g : gexec)
+  for (auto& g : gchain)
     if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
-  for (auto& row : gchain)
-    for (auto& g : row)
-      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
 }
 
 Context::~Context() {
@@ -136,19 +132,39 @@ Context::~Context() {
   if (d_partial) (void)hipFree(d_partial);
   if (d_out) (void)hipFree(d_out);
   if (h_out) (void)hipHostFree(h_out);
-  if (d_gate) (void)hipFree(d_gate);
-  if (h_gate) (void)hipHostFree(h_gate);
   if (d_igate) (void)hipFree(d_igate);
+  if (h_igate) (void)hipHostFree(h_igate);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+// The device's state words (DevLevel::stop, warm, skipc, pcur: written by
+// the gates) into the host's table, so that a re-upload of the table keeps
+// them for solves in progress (a server binds and evicts hierarchies
+// between the cycles of its running requests).  No work of the context may
+// be in flight.
+int Context::pull_state() {
+  if (!dL || hL.empty()) return AMG_OK;
+  std::vector<DevLevel> dev(hL.size());
+  AMG_HIP(hipStreamSynchronize(stream));
+  AMG_HIP(hipMemcpy(dev.data(), dL, hL.size() * sizeof(DevLevel), hipMemcpyDeviceToHost));
+  for (size_t l = 0; l < hL.size(); ++l) {
+    hL[l].stop = dev[l].stop;
+    hL[l].warm = dev[l].warm;
+    hL[l].skipc = dev[l].skipc;
+    hL[l].pcur = dev[l].pcur;
+  }
+  return AMG_OK;
+}
+
 // Removes instance I and compacts the descriptor tables to the live
 // instances' entries (re-indexed in instance order), so that a server that
 // binds and evicts hierarchies for ever keeps tables of bounded size.  The
-// device tables are re-uploaded; no work of the context may be in flight.
+// device tables are re-uploaded, with the live instances' state words; no
+// work of the context may be in flight.
 void Context::remove_instance(Instance* I) {
+  (void)pull_state();
   auto it = std::find(inst.begin(), inst.end(), I);
   if (it != inst.end()) inst.erase(it);
   delete I;
@@ -226,6 +242,10 @@ int check_grid(const amg_config& cfg, const amg_grid& g, Hierarchy& H) {
 }
 
 int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
+  {
+    const int rc = pull_state();
+    if (rc != AMG_OK) return rc;
+  }
   std::unique_ptr<Instance> hold(new Instance());
   Instance* I = hold.get();
   if (built) {
@@ -513,6 +533,15 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     AMG_HIP(hipMemcpy(I->d_met, m.data(), m.size() * 8, hipMemcpyHostToDevice));
   }
   I->bytes_per_cycle = cycle_bytes(*I);
+  I->bytes_corr = (size_t)((I->fp64 ? 16.0 : 32.0) * (double)I->H.lv[0].ncell());
+  I->bytes_vcycle = I->bytes_per_cycle - I->bytes_corr;
+  // paired cycles: mixed precision, the residual stop rule, and a finest
+  // level cycled by its own launches with a first sweep (the sweep that
+  // reads the continued iterate)
+  const int pairok = !I->fp64 && cfg.pair_cycles && cfg.stop_rule == AMG_STOP_RESIDUAL &&
+                     I->lc() >= 1 && I->pre[0] > 0;
+  for (DevLevel& d : newL) d.pairok = pairok;
+  I->pairok = pairok != 0;
   hL.insert(hL.end(), newL.begin(), newL.end());
   hX.insert(hX.end(), newX.begin(), newX.end());
   hT.push_back(tl);
@@ -616,6 +645,8 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   auto blocks_smooth = [&](const DevLevel& d) {
     return zline ? cdiv((long long)d.hx * d.ny, 64) : cdiv((long long)d.hx * d.ny * d.nz, kBlock);
   };
+  bool any_pair = false;
+  for (Instance* I : S) any_pair = any_pair || I->pairok;
   auto smooth = [&](int dpt, bool pre, bool zero_first) {
     int maxs = 0;
     for (Instance* I : S)
@@ -629,9 +660,12 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
         items.push_back({li, blocks_smooth(hL[li])});
       }
       // from a zero initial guess: the red pass sees zero everywhere, the
-      // black pass sees zero old black values (lagged periodic-z couplings)
-      half(items, 0, (zero_first && s == 0) ? 3 : 0);
-      half(items, 1, (zero_first && s == 0) ? 2 : 0);
+      // black pass sees zero old black values (lagged periodic-z couplings);
+      // the finest level of a mixed cycle that may continue the previous
+      // cycle's iterate leaves that to the instance's state word
+      const int dyn = (pre && dpt == 0 && !fp64 && any_pair) ? kZgDyn : 0;
+      half(items, 0, (zero_first && s == 0) ? 3 | dyn : 0);
+      half(items, 1, (zero_first && s == 0) ? 2 | dyn : 0);
     }
   };
   for (int dpt = 0; dpt < maxL0; ++dpt) {
@@ -741,15 +775,12 @@ std::vector<Item> march_items(const Context& C, const std::vector<Instance*>& S,
 }
 }  // namespace
 
-// Launches one cycle on every instance of S plus the norm kernels and the
-// device->host copy of the per-instance results (h_out, nvals per item);
-// gated: the gate checks record the norms on the device instead of the copy
-// (kGateOne: one instance, the context's gate; kGateEach: every instance its
-// own gate, ItemGate).
-int Context::cycle_and_norms(const std::vector<Instance*>& S, int gate_mode) {
+// Launches one cycle on every instance of S plus the norm kernels; the
+// instances' gates (ItemGate) end the cycle on the device, recording the
+// norms and deciding whether the next cycle's FP64 correction runs (paired
+// cycles); the host reads them later (sync_gates).
+int Context::cycle_and_norms(const std::vector<Instance*>& S) {
   if (S.empty()) return AMG_OK;
-  const bool gated = gate_mode == kGateOne;
-  const bool each = gate_mode == kGateEach;
   const bool fp64 = S[0]->fp64;
   const bool relmax = fp64 && cfg.report_relmax;
   if (relmax) {
@@ -757,20 +788,19 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S, int gate_mode) {
     for_lists(items, [&](const LaunchList& ll, int, int) { launch_copy_u(dL, ll, nullptr, stream); });
   }
   vcycle(S, fp64);
-  auto items = march_items(*this, S, true);
+  // the FP64 finest entry 0: the kernels take the current pressure buffer
+  // from the instance's state word pcur
+  auto items = march_items(*this, S, false);
   const int nv = 4;
-  // gated (one instance): the cycle's last norm fold also ends the cycle
-  const GateArgs ga{d_gate, d_out, d_out + kOutRegion * 2, dL, (int)hL.size(), S[0]->id};
-  const GateArgs* gate_res = (gated && !relmax) ? &ga : nullptr;
   // per-instance gates: item t of a list at offset ioff
   auto each_args = [&](int ioff) {
-    return GateArgs{nullptr, d_out + (size_t)ioff * nv, d_out + kOutRegion * 2 + (size_t)ioff * 2,
-                    dL, (int)hL.size(), 0, d_igate};
+    return GateArgs{d_out + (size_t)ioff * nv, d_out + kOutRegion * 2 + (size_t)ioff * 2, dL,
+                    (int)hL.size(), d_igate};
   };
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     double* part = d_partial + (size_t)boff * nv;
     const GateArgs ge = each_args(ioff);
-    const GateArgs* gr = (each && !relmax) ? &ge : gate_res;
+    const GateArgs* gr = relmax ? nullptr : &ge;
     if (fp64) {
       launch_residual_max(dL, ll, part, stream);
       launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream, gr);
@@ -785,24 +815,19 @@ int Context::cycle_and_norms(const std::vector<Instance*>& S, int gate_mode) {
       double* part = d_partial + (size_t)boff * nv;
       const GateArgs ge = each_args(ioff);
       launch_change_max(dL, ll, part, stream);
-      launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream,
-                      gated ? &ga : each ? &ge : nullptr);
+      launch_finalize(ll, part, 2, 0, d_out + kOutRegion * 2 + (size_t)ioff * 2, 2, stream, &ge);
     });
   }
-  if (gated || each) return AMG_OK;
-  (void)hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream);
-  if (relmax)
-    (void)hipMemcpyAsync(h_out + kOutRegion * 2, d_out + kOutRegion * 2, sizeof(double) * S.size() * 2,
-                         hipMemcpyDeviceToHost, stream);
   return AMG_OK;
 }
 
-// Reads the results of cycle_and_norms (after the stream is synchronised).
-void Context::post_norms(const std::vector<Instance*>& S) {
+// The host's norm step for one correction closing m V-cycles, from h_out
+// (and its relmax region): the stop rule of mgMK/mg_solver.F90:114-131.
+void Context::post_norms(const std::vector<Instance*>& S, int m) {
   const int nv = 4;
   for (size_t q = 0; q < S.size(); ++q) {
     Instance* I = S[q];
-    I->iters += 1;
+    I->iters += m;
     const double* o = h_out + q * nv;
     const double rmax = o[0];
     double relmax = 0.0;
@@ -863,74 +888,30 @@ const double* Context::null_sums(const std::vector<Instance*>& S) {
   return c;
 }
 
-int Context::prepare(const std::vector<Instance*>& S) {
+// The start of a solve for every instance of S, enqueued only: the state
+// words cleared, the Fortran arrays in the chequer layout (b with the
+// boundary fold) with sum V b, the first defect (mixed: also the FP32 defect
+// of the first cycle; for a singular operator the incompatible part of b is
+// removed first, in place, b -= sum V b / sum V), and the gates initialised
+// from it on the device.
+int Context::prepare_async(const std::vector<Instance*>& S) {
   const int nv = 4;
   PtrList rhs{}, phi{};
-  std::vector<Item> items;
-  for (size_t q = 0; q < S.size(); ++q) {
-    Instance* I = S[q];
-    I->cur = 0;
-    I->iters = 0;
-    I->converged = false;
-    I->done = false;
-    I->status = AMG_OK;
-    items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, (long long)kBlock * kWsumPer)});
-  }
-  // Fortran arrays -> chequer FP64 (b with the boundary fold), and sum V b
-  double* c = d_out + kOutRegion * 3;
-  for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
-    for (int t = 0; t < ll.nitems; ++t) {
-      rhs.p[t] = const_cast<double*>(S[ioff + t]->d_rhs);
-      phi.p[t] = S[ioff + t]->d_phi;
-    }
-    launch_ingest(dL, ll, rhs, phi, d_partial + boff, stream);
-    launch_finalize(ll, d_partial + boff, 1, 1, c + ioff, 1, stream);
-  });
-  // initial residual (mixed: also the FP32 defect of the first cycle); for a
-  // singular operator the incompatible part of b is removed first, in place
-  // (b -= sum V b / sum V)
-  const bool fp64 = S[0]->fp64;
-  const auto fitems = march_items(*this, S, false);
-  for_lists(fitems, [&](const LaunchList& ll, int ioff, int boff) {
-    double* part = d_partial + (size_t)boff * nv;
-    launch_first_defect(fp64, dL, ll, c + ioff, part, stream);
-    launch_finalize(ll, part, 2, 0, d_out + (size_t)ioff * nv, nv, stream);
-  });
-  AMG_HIP(hipMemcpyAsync(h_out, d_out, sizeof(double) * S.size() * nv, hipMemcpyDeviceToHost, stream));
-  AMG_HIP(hipStreamSynchronize(stream));
-  for (size_t q = 0; q < S.size(); ++q) {
-    Instance* I = S[q];
-    const double r = h_out[q * nv], b = h_out[q * nv + 1];
-    I->bnorm = b;
-    I->res0 = b > 0.0 ? r / b : r;
-    I->relres = I->res0;
-    I->relmax = 0.0;
-    if (!std::isfinite(r) || !std::isfinite(b)) {
-      I->status = AMG_ENAN;
-      I->done = true;
-    } else if (cfg.stop_rule == AMG_STOP_RESIDUAL && (b == 0.0 || I->res0 <= cfg.tol)) {
-      I->converged = true;
-      I->done = true;
-    }
-  }
-  return AMG_OK;
-}
-
-int Context::prepare_gated(const std::vector<Instance*>& S) {
-  const int nv = 4;
-  PtrList rhs{}, phi{};
-  std::vector<Item> items;
+  std::vector<Item> items, mine;
   for (Instance* I : S) {
     I->cur = 0;
     I->iters = 0;
+    I->dev_cycles = 0;
     I->converged = false;
     I->done = false;
     I->status = AMG_OK;
     I->gate_seen = false;
-    // the stop words of the instance's previous solve
-    launch_gate_reset(dL, (int)hL.size(), I->id, stream);
     items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, (long long)kBlock * kWsumPer)});
+    mine.push_back({I->fine[0], 1});
   }
+  for_lists(mine, [&](const LaunchList& ll, int, int) {
+    launch_gate_reset(dL, (int)hL.size(), ll, stream);
+  });
   double* c = d_out + kOutRegion * 3;
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     for (int t = 0; t < ll.nitems; ++t) {
@@ -953,9 +934,33 @@ int Context::prepare_gated(const std::vector<Instance*>& S) {
   return AMG_OK;
 }
 
-int Context::absorb_gate(Instance* I, const ItemGate& g) {
+int Context::gate_slots() const {
+  int n = 0;
+  for (const Instance* I : inst) n = std::max(n, I->gslot + 1);
+  return n;
+}
+
+// Brings the gates of the instances of S to the host and replays them.
+int Context::sync_gates(const std::vector<Instance*>& S, double* bytes) {
+  AMG_HIP(hipMemcpyAsync(h_igate, d_igate, sizeof(ItemGate) * gate_slots(), hipMemcpyDeviceToHost,
+                         stream));
+  AMG_HIP(hipStreamSynchronize(stream));
+  for (Instance* I : S) {
+    const int rc = absorb_gate(I, h_igate[I->gslot], bytes);
+    if (rc != AMG_OK) return rc;
+  }
+  return AMG_OK;
+}
+
+int Context::prepare(const std::vector<Instance*>& S) {
+  int rc = prepare_async(S);
+  if (rc != AMG_OK) return rc;
+  return sync_gates(S, nullptr);
+}
+
+int Context::absorb_gate(Instance* I, const ItemGate& g, double* bytes) {
   if (!I->gate_seen) {
-    // the first defect (Context::prepare's decisions)
+    // the first defect (the decisions of k_gate_init)
     I->gate_seen = true;
     I->bnorm = g.bnorm;
     I->res0 = g.res0;
@@ -968,25 +973,34 @@ int Context::absorb_gate(Instance* I, const ItemGate& g) {
       I->converged = true;
       I->done = true;
     }
+    I->units = 0;
   }
-  if (g.count < I->iters || g.count - I->iters > kItemRing) {
-    set_error("served round: the host lost track of the device's cycles");
+  if (g.count < I->units || g.count - I->units > kItemRing) {
+    set_error("gated cycles: the host lost track of the device's corrections");
     return AMG_EHIP;
   }
   const std::vector<Instance*> A{I};
-  for (int c = I->iters; c < g.count; ++c) {
+  for (int c = I->units; c < g.count; ++c) {
     if (I->done) {
-      set_error("served round: the device ran past the host's stop");
+      set_error("gated cycles: the device ran past the host's stop");
       return AMG_EHIP;
     }
     const double* h = g.hist[c % kItemRing];
+    const int m = g.hcyc[c % kItemRing];
+    if (m < 1 || m > 2) {
+      set_error("gated cycles: a correction closing " + std::to_string(m) + " cycles");
+      return AMG_EHIP;
+    }
     for (int q = 0; q < 4; ++q) h_out[q] = h[q];
     h_out[kOutRegion * 2] = h[4];
     h_out[kOutRegion * 2 + 1] = h[5];
-    post_norms(A);
+    post_norms(A, m);
+    if (bytes) *bytes += (double)(m * I->bytes_vcycle + I->bytes_corr);
   }
+  I->units = g.count;
+  I->dev_cycles = g.cycles;
   if ((g.stop != 0) != I->done) {
-    set_error("served round: the device and the host stop at different cycles");
+    set_error("gated cycles: the device and the host stop at different cycles");
     return AMG_EHIP;
   }
   return AMG_OK;
@@ -1023,89 +1037,45 @@ int Context::finish(const std::vector<Instance*>& S) {
 
 // The cycles of one instance in gated batches: a batch enqueues as many
 // cycles as the instance's previous solve took (one when there is none)
-// without a host round trip between them; each cycle ends with a gate
-// check (in its last norm fold), and once it stops the batch, the rest of
-// the batch's kernels return at once.  After the batch the host replays the
-// recorded norms through post_norms, which must reach the same decision at
-// the same cycle.  A batch runs as graphs of 4, 2 and 1 consecutive cycles
-// (the fewest graph launches: each graph boundary costs the device several
-// microseconds), captured with the gate in their launch lists, per chain
-// length and, in mixed precision, per parity of the current FP64 buffer at
-// the chain's start.  When capture is unavailable the caller's ungated loop
-// takes over.
+// without a host round trip between them; each cycle ends with its gate
+// (ItemGate), and once it stops, the rest of the batch's kernels return at
+// once.  After the batch the host replays the recorded corrections.  A
+// batch runs as graphs of 4, 2 and 1 consecutive cycles (the fewest graph
+// launches: each graph boundary costs the device several microseconds),
+// captured once: the launch lists do not depend on the solve's state, which
+// the kernels read from the instance's state words.  When capture is
+// unavailable the caller's loop takes over.
 int Context::gated_cycles(Instance* I, double* bytes) {
   const std::vector<Instance*> A{I};
-  while (!I->done && I->iters < cfg.maxcy) {
-    int batch = I->last_iters > I->iters ? I->last_iters - I->iters : 1;
-    batch = std::min(batch, std::min(kGateHist, cfg.maxcy - I->iters));
-    // the batch as chains (length index, graph index), longest first
-    std::vector<std::pair<int, int>> plan;
-    for (int left = batch, par = I->cur; left > 0;) {
+  while (!I->done) {
+    int batch = I->last_iters > I->dev_cycles ? I->last_iters - I->dev_cycles : 1;
+    batch = std::max(1, std::min(batch, std::min(kItemRing, cfg.maxcy - I->dev_cycles)));
+    std::vector<int> plan;
+    for (int left = batch; left > 0;) {
       const int q = left >= 4 ? 2 : left >= 2 ? 1 : 0;
-      plan.push_back({q, I->fp64 ? 0 : 1 + par});
+      plan.push_back(q);
       left -= kChainLen[q];
-      if (!I->fp64) par = (par + kChainLen[q]) & 1;
     }
-    for (const auto& [q, gi] : plan) {
-      if (gchain[q][gi]) continue;
-      const int cur0 = I->cur;
-      if (!I->fp64) I->cur = gi - 1;
+    for (int q : plan) {
+      if (gchain[q]) continue;
       hipGraph_t g = nullptr;
       bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
       if (ok) {
-        for (int c = 0; c < kChainLen[q]; ++c) {
-          cycle_and_norms(A, kGateOne);
-          if (!I->fp64) I->cur ^= 1;     // as post_norms does after a mixed cycle
-        }
+        for (int c = 0; c < kChainLen[q]; ++c) cycle_and_norms(A);
         ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
       }
-      if (ok) ok = hipGraphInstantiate(&gchain[q][gi], g, nullptr, nullptr, 0) == hipSuccess;
+      if (ok) ok = hipGraphInstantiate(&gchain[q], g, nullptr, nullptr, 0) == hipSuccess;
       if (g) (void)hipGraphDestroy(g);
-      I->cur = cur0;
       if (!ok) {
         graph_ok = false;
-        gchain[q][gi] = nullptr;
+        gchain[q] = nullptr;
         (void)hipGetLastError();
         return AMG_OK;
       }
     }
-    CycleGate& h = *h_gate;
-    h.stop = 0;
-    h.count = 0;
-    h.iters0 = I->iters;
-    h.maxcy = cfg.maxcy;
-    h.rule = cfg.stop_rule;
-    h.fp64 = I->fp64 ? 1 : 0;
-    h.report = (I->fp64 && cfg.report_relmax) ? 1 : 0;
-    h.pad = 0;
-    h.bnorm = I->bnorm;
-    h.tol = cfg.tol;
-    AMG_HIP(hipMemcpyAsync(d_gate, h_gate, offsetof(CycleGate, hist), hipMemcpyHostToDevice, stream));
-    for (const auto& [q, gi] : plan) AMG_HIP(hipGraphLaunch(gchain[q][gi], stream));
-    AMG_HIP(hipMemcpyAsync(h_gate, d_gate, sizeof(CycleGate), hipMemcpyDeviceToHost, stream));
-    launch_gate_reset(dL, (int)hL.size(), I->id, stream);
-    AMG_HIP(hipStreamSynchronize(stream));
-    if (h.count < 1 || h.count > batch) {
-      set_error("gated cycles: the device ran " + std::to_string(h.count) + " of " +
-                std::to_string(batch) + " cycles");
-      return AMG_EHIP;
-    }
-    for (int c = 0; c < h.count; ++c) {
-      if (I->done) {
-        set_error("gated cycles: the device ran past the host's stop");
-        return AMG_EHIP;
-      }
-      for (int q = 0; q < 4; ++q) h_out[q] = h.hist[c][q];
-      h_out[kOutRegion * 2] = h.hist[c][4];
-      h_out[kOutRegion * 2 + 1] = h.hist[c][5];
-      post_norms(A);
-      *bytes += (double)I->bytes_per_cycle;
-    }
-    const bool full = h.count == kGateHist;
-    if ((h.stop != 0) != I->done && !(full && !I->done)) {
-      set_error("gated cycles: the device and the host stop at different cycles");
-      return AMG_EHIP;
-    }
+    for (int q : plan) AMG_HIP(hipGraphLaunch(gchain[q], stream));
+    const int rc = sync_gates(A, bytes);
+    if (rc != AMG_OK) return rc;
   }
   return AMG_OK;
 }
@@ -1116,48 +1086,39 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
   int rc = prepare(S);
   if (rc != AMG_OK) return rc;
   AMG_HIP(hipEventRecord(ev[0], stream));
-  const bool single = S.size() == 1 && cfg.use_graph;
   double bytes = 0.0;
-  if (single && graph_ok) {
+  if (S.size() == 1 && cfg.use_graph && graph_ok) {
     rc = gated_cycles(S[0], &bytes);
     if (rc != AMG_OK) return rc;
   }
-  for (int it = 0; it < cfg.maxcy; ++it) {
+  // one gated cycle of every running instance at a time
+  for (int it = 0; it <= cfg.maxcy; ++it) {
     std::vector<Instance*> A;
     for (Instance* I : S)
       if (!I->done) A.push_back(I);
     if (A.empty()) break;
-    if (single && graph_ok) {
-      Instance* I = A[0];
-      const int gi = I->fp64 ? 0 : 1 + I->cur;
-      if (!gexec[gi]) {
-        hipGraph_t g = nullptr;
-        bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (ok) {
-          cycle_and_norms(A, kGateNone);
-          ok = hipStreamEndCapture(stream, &g) == hipSuccess && g;
-        }
-        if (ok) ok = hipGraphInstantiate(&gexec[gi], g, nullptr, nullptr, 0) == hipSuccess;
-        if (g) (void)hipGraphDestroy(g);
-        if (!ok) {
-          graph_ok = false;
-          gexec[gi] = nullptr;
-          (void)hipGetLastError();
-        }
-      }
-      if (gexec[gi]) AMG_HIP(hipGraphLaunch(gexec[gi], stream));
-      else cycle_and_norms(A, kGateNone);
-    } else {
-      cycle_and_norms(A, kGateNone);
-    }
-    AMG_HIP(hipStreamSynchronize(stream));
-    post_norms(A);
-    for (Instance* I : A) bytes += (double)I->bytes_per_cycle;
+    rc = cycle_and_norms(A);
+    if (rc == AMG_OK) rc = sync_gates(A, &bytes);
+    if (rc != AMG_OK) return rc;
   }
+  for (Instance* I : S)
+    if (!I->done) {
+      set_error("gated cycles: the device did not stop within maxcy cycles");
+      return AMG_EHIP;
+    }
   AMG_HIP(hipEventRecord(ev[1], stream));
   for (Instance* I : S) I->last_iters = I->iters;
   rc = finish(S);
   if (rc != AMG_OK) return rc;
+  // the state words back to running (kernels launched on the levels later,
+  // e.g. run_kernel, must not find a stop)
+  {
+    std::vector<Item> mine;
+    for (Instance* I : S) mine.push_back({I->fine[0], 1});
+    for_lists(mine, [&](const LaunchList& ll, int, int) {
+      launch_gate_reset(dL, (int)hL.size(), ll, stream);
+    });
+  }
   AMG_HIP(hipEventRecord(ev[3], stream));
   AMG_HIP(hipStreamSynchronize(stream));
   AMG_HIP(hipGetLastError());

@@ -17,8 +17,6 @@ void set_error(const std::string& msg);
 const std::string& last_error();
 
 constexpr int kMaxInstances = 1024;
-// gate modes of Context::cycle_and_norms
-enum { kGateNone = 0, kGateOne = 1, kGateEach = 2 };
 
 #define AMG_HIP(call)                                                        \
   do {                                                                       \
@@ -64,6 +62,7 @@ struct Instance {
   bool tail_222 = true;          //   and uniform 2:1 transfers
   bool tail_split = false;       // the coarse solve is its own kernel (large coarsest level)
   int fine[2] = {-1, -1};        // FP64 finest entries (u = p_cur for cur = 0/1)
+  bool pairok = false;           // may pair cycles (DevLevel::pairok)
   int cur = 0;                   // which fine entry holds the current pressure
   std::vector<int> pre, post;
   DevArena mem;
@@ -75,8 +74,11 @@ struct Instance {
   int last_iters = 0;            // cycles of the previous solve (the gated batch size)
   double bnorm = 0.0, relres = 0.0, relmax = 0.0, res0 = 0.0;
   int status = AMG_OK;
-  bool gate_seen = false;        // served: the host has read the gate of the first defect
-  size_t bytes_per_cycle = 0;
+  bool gate_seen = false;        // the host has read the gate of the first defect
+  int units = 0;                 // corrections replayed from the gate
+  int dev_cycles = 0;            // V-cycles the device had run at the last look
+  size_t bytes_per_cycle = 0;    // a V-cycle and its FP64 correction
+  size_t bytes_vcycle = 0, bytes_corr = 0;   // the two parts
   double* d_phi = nullptr;       // caller arrays of the current solve
   const double* d_rhs = nullptr;
   size_t device_bytes() const { return mem.bytes; }
@@ -99,15 +101,12 @@ struct Context {
   size_t cap_partial = 0;
   double* d_out = nullptr;       // per-item results
   double* h_out = nullptr;       // pinned mirror
-  CycleGate* d_gate = nullptr;   // gated cycles: device control block
-  CycleGate* h_gate = nullptr;   // and its pinned mirror
-  ItemGate* d_igate = nullptr;   // per-instance gates (served rounds), kMaxInstances
+  ItemGate* d_igate = nullptr;   // per-instance gates, kMaxInstances
+  ItemGate* h_igate = nullptr;   // and their pinned mirror
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // single-instance graphs: [0] fp64 cycle, [1]/[2] mixed cycle for cur = 0/1
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-  // gated single-instance graphs of kChainLen[q] cycles, indexed as gexec
+  // gated single-instance graphs of kChainLen[q] cycles
   static constexpr int kChainLen[3] = {1, 2, 4};
-  hipGraphExec_t gchain[3][3] = {};
+  hipGraphExec_t gchain[3] = {};
   void drop_graphs();
   bool graph_ok = true;
 
@@ -116,6 +115,7 @@ struct Context {
   // builds the hierarchy (or takes `built`, from check_grid) and allocates
   int add_instance(const amg_grid& g, Instance** out, Hierarchy* built = nullptr);
   int upload_tables();
+  int pull_state();
   int solve(const std::vector<Instance*>& S, amg_stats* st);
   // pieces
   void vcycle(const std::vector<Instance*>& S, bool fp64);
@@ -123,18 +123,21 @@ struct Context {
   void direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes, size_t lds,
                        bool fp64);
   int prepare(const std::vector<Instance*>& S);
+  // the start of a solve (prepare) enqueued only: the gates of S are
+  // initialised on the device, the host reads them later
+  int prepare_async(const std::vector<Instance*>& S);
+  int gate_slots() const;
+  // copies the gates to the host, waits, and replays those of S
+  int sync_gates(const std::vector<Instance*>& S, double* bytes);
   const double* null_sums(const std::vector<Instance*>& S);
   int finish(const std::vector<Instance*>& S);
-  int cycle_and_norms(const std::vector<Instance*>& S, int gate_mode);
-  // the start of a solve for a served round: like prepare, with the gates
-  // of S initialised on the device instead of a host round trip
-  int prepare_gated(const std::vector<Instance*>& S);
-  // replays the cycles that instance I ran on the device since the host's
-  // last look (its gate as copied after a round); AMG_EHIP when the device
-  // and the host disagree
-  int absorb_gate(Instance* I, const ItemGate& g);
+  int cycle_and_norms(const std::vector<Instance*>& S);
+  // replays the corrections that instance I made on the device since the
+  // host's last look (its gate as copied after a batch or round), adding
+  // their bytes; AMG_EHIP when the device and the host disagree
+  int absorb_gate(Instance* I, const ItemGate& g, double* bytes);
   int gated_cycles(Instance* I, double* bytes);
-  void post_norms(const std::vector<Instance*>& S);
+  void post_norms(const std::vector<Instance*>& S, int m);
   int run_kernel(Instance* I, int which, int level, int reps, double* bytes);
   int level_field(Instance* I, int level, int field, int dir, double* buf);
   void remove_instance(Instance* I);

@@ -33,6 +33,7 @@ module amg_cabi
      integer(c_int) :: strict
      integer(c_int) :: use_graph
      integer(c_int) :: coarse_direct
+     integer(c_int) :: pair_cycles
   end type amg_config
 
   type, bind(C) :: amg_grid

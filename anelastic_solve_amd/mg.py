@@ -30,7 +30,7 @@ def default_config(**kw):
     'relmax'), pre, post (int or coarsest-first list), tbc (6 ints, face
     order X2max, X3min, X2min, X3max, X1min, X1max), vbc (6 floats),
     precision ('fp64'|'mixed'), nlevels (3 ints), report_relmax, verbose,
-    use_graph, coarse_direct.
+    use_graph, coarse_direct, pair_cycles.
     """
     lib = _capi.load()
     c = _capi.amg_config()
@@ -56,7 +56,8 @@ def default_config(**kw):
         elif k == "nlevels":
             for i in range(3):
                 c.nlevels[i] = int(v[i])
-        elif k in ("maxcy", "report_relmax", "verbose", "use_graph", "strict", "coarse_direct"):
+        elif k in ("maxcy", "report_relmax", "verbose", "use_graph", "strict", "coarse_direct",
+                   "pair_cycles"):
             setattr(c, k, int(v))
         elif k == "tol":
             c.tol = float(v)

@@ -92,6 +92,12 @@ typedef struct amg_config {
                                    FP64) when z is not periodic and the level is above
                                    the coarse tail; the levels below it are not cycled.
                                    0: cycle every level down to the coarsest           */
+  int    pair_cycles;           /* 1 (default): mixed precision with the residual stop
+                                   rule may close two FP32 V-cycles with one FP64
+                                   correction when the solve's own convergence rate
+                                   says one more cycle cannot reach tol (the device
+                                   decides after every correction); 0: one correction
+                                   per V-cycle                                          */
 } amg_config;
 
 /*

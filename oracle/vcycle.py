@@ -310,8 +310,28 @@ def maxabs(a):
     return float(np.max(np.abs(a))) if a.size else 0.0
 
 
+PAIR_RHO0 = 1e-3
+
+
+def pair_next(r, rp, m, tol):
+    """Whether the next FP64 correction should close two FP32 V-cycles.
+
+    ``r``: the relative residual after the last correction, ``rp`` the one
+    before it, ``m`` V-cycles earlier.  The per-cycle rate is ``rho = (r /
+    rp)^(1/m)``; two cycles are paired when one cannot reach the tolerance,
+    ``r * rho > tol``, decided with multiplications only (``m = 2``:
+    ``r^2 (r / rp) > tol^2``).  This is the product's paired-cycle rule
+    (its device gate and host replay); it has no counterpart in the
+    reference, whose cycles are all FP64.
+    """
+    if not rp > 0.0:
+        return False
+    q = r / rp
+    return r * q > tol if m == 1 else (r * r) * q > tol * tol
+
+
 def solve(levels, factors, tbc, vbc, p0, b_in, *, pre=2, post=2, smoother=LJ,
-          mixed=False, tol=1e-10, maxcy=200, remove_null=True, direct_level=None):
+          mixed=False, tol=1e-10, maxcy=200, remove_null=True, direct_level=None, pair=False):
     """Solve to ``||b - A p||_inf / ||b||_inf <= tol`` (driver of ``mgMK/mg_solver.F90``).
 
     ``mixed=False``: V-cycles on ``p`` in FP64.  ``mixed=True``: FP64 defect
@@ -321,6 +341,10 @@ def solve(levels, factors, tbc, vbc, p0, b_in, *, pre=2, post=2, smoother=LJ,
     max|p_new - p_old| / max|p_new|`` history (``mgMK/mg_residual.F90:60-94``).
     ``direct_level``: cycle the levels above it and solve it exactly
     (``separable_solve``) instead of going down to the coarsest level.
+    ``pair`` (mixed): a correction may close two V-cycles, the second
+    continuing from the first's ``e`` (``pair_next``; the first correction
+    pairs when ``res0 * PAIR_RHO0 > tol``); the histories then hold one entry
+    per correction and ``info["cycles"]`` the V-cycles each closes.
     """
     singular = is_singular(tbc)
     if direct_level is not None:
@@ -339,26 +363,35 @@ def solve(levels, factors, tbc, vbc, p0, b_in, *, pre=2, post=2, smoother=LJ,
         Minv = coarse_operator_inverse(levels[-1], singular)
     p = np.array(p0, dtype=np.float64, copy=True)
     bn = maxabs(b)
-    res_hist, rel_hist = [], []
+    res_hist, rel_hist, cyc_hist = [], [], []
     r = residual(p, b, fine)
     it = 0
     conv = maxabs(r) <= tol * bn
+    rprev = maxabs(r) / bn if bn > 0 else maxabs(r)
+    two = pair and mixed and maxcy >= 2 and rprev * PAIR_RHO0 > tol
     while not conv and it < maxcy:
-        it += 1
         p_old = p.copy()
+        m = 2 if two else 1
         if mixed:
             e = np.zeros(p.shape, dtype=np.float32)
-            vcycle(levels, factors, e, r.astype(np.float32), pre_l, post_l, smoother,
-                   np.float32, Minv, 0, zero_guess=True)
+            r32 = r.astype(np.float32)
+            for q in range(m):
+                vcycle(levels, factors, e, r32, pre_l, post_l, smoother, np.float32, Minv, 0,
+                       zero_guess=(q == 0))
             p = p + e.astype(np.float64)
         else:
             vcycle(levels, factors, p, b, pre_l, post_l, smoother, np.float64, Minv, 0)
+        it += m
         r = residual(p, b, fine)
         res_hist.append(maxabs(r) / bn if bn > 0 else 0.0)
+        cyc_hist.append(m)
         pm = maxabs(p)
         rel_hist.append(maxabs(p - p_old) / pm if pm > 0 else 0.0)
         conv = res_hist[-1] <= tol
+        two = (pair and mixed and not conv and it + 2 <= maxcy
+               and pair_next(res_hist[-1], rprev, m, tol))
+        rprev = res_hist[-1]
     if singular and remove_null:
         p = null_project(p, fine)
     return p, {"iters": it, "residual": res_hist, "relmax": rel_hist, "converged": conv,
-               "b": b}
+               "cycles": cyc_hist, "b": b}

@@ -296,3 +296,46 @@ def test_split_coarse_tail(cuda):
     g, b = grids[0], rhs[0].cpu().numpy()
     x, _ = _direct(g, b, tuple(cfg().tbc), None)     # the finest operator does not depend on the levels
     assert relerr(phis[0].cpu().numpy()[1:-1, 1:-1, 1:-1], x) <= 1e-9
+
+
+@pytest.mark.parametrize("smoother", ["LJ", "RBPGS"])
+def test_paired_cycles(cuda, smoother):
+    """Mixed precision with paired cycles (two FP32 V-cycles closed by one
+    FP64 correction, decided on the device from the solve's own rate) stops
+    after as many V-cycles as one correction per cycle, and both reach the direct
+    solution to 1e-9; repeated paired solves give the same bits.  The
+    oracle's paired solve (the same rule in numpy) stops within one cycle of
+    the device's and closes fewer corrections than cycles."""
+    torch = cuda
+    from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
+    from oracle import vcycle as VC
+    g = box_grid(32, 32, 32, beta=2.0)
+    b = workloads.psource_host(*workloads.velocity_white(g, 61), g)
+    tbc = (0, 1, 0, 1, 0, 0)
+    x, levs = _direct(g, b, tbc, None)
+    its, sols = {}, {}
+    for pair in (0, 1):
+        s = PressureSolver(g, default_config(smoother=smoother, tol=1e-12, maxcy=400,
+                                             pair_cycles=pair))
+        outs = []
+        for rep in range(2):
+            p = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
+            it, _ = s.mg_solver(p, torch.from_numpy(b).cuda())
+            assert s.stats.converged == 1 and s.stats.relres <= 1e-12
+            outs.append(p)
+        assert torch.equal(outs[0], outs[1])
+        its[pair], sols[pair] = it, outs[0].cpu().numpy()[1:-1, 1:-1, 1:-1]
+        dl = s.direct_level
+        s.close()
+        assert relerr(sols[pair], x) <= 1e-9, (pair, relerr(sols[pair], x))
+    assert its[1] == its[0], its
+    # the oracle's paired solve: the same number of cycles
+    from oracle import ops
+    bb = ops.compat_project(b[1:-1, 1:-1, 1:-1], levs[0])
+    _, info = VC.solve(levs, oracle_levels(g, tbc)[1], list(tbc), [0.0] * 6, np.zeros_like(bb), bb,
+                       smoother=smoother, mixed=True, tol=1e-12, maxcy=400,
+                       direct_level=dl if dl >= 1 else None, pair=True)
+    print(f"{smoother}: GPU {its[1]} cycles paired/{its[0]} not; oracle paired {info['iters']} "
+          f"({info['cycles']})")
+    # FP32 rounding differs between numpy and the device: the stop may move by one cycle
+    assert abs(info["iters"] - its[1]) <= 1 and len(info["cycles"]) < info["iters"]

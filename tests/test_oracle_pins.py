@@ -429,3 +429,51 @@ def test_vcycle_with_direct_level_converges_to_exact():
                            mixed=mixed, tol=1e-12, direct_level=2)
         assert info["converged"] and info["iters"] <= 12, info["iters"]
         assert np.abs(p - x).max() <= 1e-9 * np.abs(x).max()
+
+
+def test_paired_cycles_keep_the_per_cycle_rate():
+    """Closing two FP32 V-cycles with one FP64 correction (the second cycle
+    continuing from the first's correction e) leaves the residual after
+    every even number of cycles where one correction per cycle puts it (the
+    FP32 cycle's rate does not depend on the FP64 update between cycles),
+    so the paired solve stops after as many V-cycles with fewer corrections
+    and reaches the same answer.  The rule pairs only when one more cycle
+    is predicted not to reach the tolerance from the solve's own rate, so
+    no pair runs a cycle past the one where the unpaired solve stops."""
+    levs, fac, _ = _levels((32, 32, 32), beta=2.0)
+    rng = np.random.default_rng(12)
+    b = ops.compat_project(rng.standard_normal((32, 32, 32)), levs[0])
+    x = direct.solve(levs[0], b, [0, 1, 0, 1, 0, 0])
+    run = {}
+    for pair in (False, True):
+        run[pair] = VC.solve(levs, fac, [0, 1, 0, 1, 0, 0], [0.0] * 6, np.zeros_like(b), b,
+                             mixed=True, tol=1e-10, direct_level=2, pair=pair)
+    one, two = run[False][1], run[True][1]
+    assert one["converged"] and two["converged"]
+    assert two["iters"] == one["iters"]
+    assert len(two["residual"]) < len(one["residual"])
+    assert two["cycles"][0] == 2 and sum(two["cycles"]) == two["iters"]
+    # the residual after the same number of V-cycles, paired or not
+    done = np.cumsum(two["cycles"])
+    for c, m, r in zip(done, two["cycles"], two["residual"]):
+        ref = one["residual"][c - 1]
+        assert 0.5 * ref <= r <= 2.0 * ref, (c, r, ref)
+        if m == 2:      # the first cycle of the pair would not have stopped the solve
+            assert one["residual"][c - 2] > 1e-10, (c, one["residual"][c - 2])
+    for p, _ in run.values():
+        assert np.abs(p - x).max() <= 1e-9 * np.abs(x).max()
+
+
+def test_pair_rule_cases():
+    """pair_next: pairs exactly when the residual one cycle on, at the rate
+    of the last correction, is still above the tolerance."""
+    tol = 1e-10
+    # one cycle per correction: rate 0.1 from 1e-8 -> 1e-9 > tol: pair
+    assert VC.pair_next(1e-8, 1e-7, 1, tol)
+    # rate 0.01 from 1e-8 -> 1e-10, not above the tolerance: one cycle
+    assert not VC.pair_next(1e-8, 1e-6, 1, tol)
+    # two cycles: rate sqrt(1e-2) = 0.1 from 1e-8 -> 1e-9: pair
+    assert VC.pair_next(1e-8, 1e-6, 2, tol)
+    # two cycles: rate sqrt(1e-4) = 0.01 from 1e-8 -> 1e-10: one cycle
+    assert not VC.pair_next(1e-8, 1e-4, 2, tol)
+    assert not VC.pair_next(1e-8, 0.0, 1, tol)
