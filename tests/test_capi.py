@@ -46,6 +46,7 @@ def test_config_defaults():
     assert c.smoother == _capi.SMOOTHER_LJ and c.maxcy == 100 and c.tol == 1e-10
     assert list(c.tbc) == [0, 1, 0, 1, 0, 0]
     assert c.precision == _capi.MIXED and list(c.pre_relax)[:4] == [2, 2, 2, 2]
+    assert c.coarse_direct == 1 and c.pair_cycles == 1
 
 
 INI = """
@@ -122,6 +123,21 @@ def test_tolres_selects_residual_rule(tmp_path):
     c, stat = A.mg_read_params(p)
     assert stat == 0 and c.stop_rule == _capi.STOP_RESIDUAL and c.tol == 1e-10
     assert c.precision == _capi.FP64
+
+
+def test_coarse_direct_and_pair_cycles_keys(tmp_path):
+    """The two optional [MGsolver] switches of this build: yes/no as the
+    reference's logical keys (Strict, Verbose), anything else a parse error."""
+    p = tmp_path / "k.ini"
+    p.write_text(INI.replace("tolmax=1.0d-4", "tolmax=1.0d-4\nCoarseDirect=no\nPairCycles=no"))
+    c, stat = A.mg_read_params(p)
+    assert stat == 0 and c.coarse_direct == 0 and c.pair_cycles == 0
+    p.write_text(INI)
+    c, stat = A.mg_read_params(p)
+    assert stat == 0 and c.coarse_direct == 1 and c.pair_cycles == 1
+    p.write_text(INI.replace("tolmax=1.0d-4", "tolmax=1.0d-4\nPairCycles=sometimes"))
+    _, stat = A.mg_read_params(p)
+    assert stat == _capi.AMG_EPARSE
 
 
 def test_errors_have_names():
