@@ -187,6 +187,17 @@ void launch_fill_zero(bool fp64, const DevLevel* L, const LaunchList& ll, hipStr
 // ---- V-cycle kernels (amg_kernels.hip); fp64 selects the level precision --
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                      int zero_guess, hipStream_t s);
+// The point half-sweep marching in z over slabs of 'planes' (8, 4 or 2)
+// planes: cdiv(hx*ny, kBlock) * cdiv(nz, planes) blocks per item.
+void launch_rb_march(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
+                     int zero_guess, int planes, hipStream_t s);
+// Slab length of the marching point sweep for a level (0: one thread per
+// cell): the longest of 8, 4, 2 planes that keeps 2^18 threads in a launch.
+inline int rb_march_planes(int hx, int ny, int nz) {
+  for (int K : {8, 4, 2})
+    if (nz >= K && (long long)hx * ny * ((nz + K - 1) / K) >= (1 << 18)) return K;
+  return 0;
+}
 // z-line sweeps run 64 columns per workgroup and 'segs' = ceil(nz_max/kZSeg)
 // waves (segments of kZSeg cells) per column group; every item of one
 // launch must have nz <= segs*kZSeg.

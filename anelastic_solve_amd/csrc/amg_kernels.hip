@@ -254,6 +254,95 @@ __global__ __launch_bounds__(kBlock) void k_rb_point(const DevLevel* __restrict_
                      zero_flags(L, zero_guess));
 }
 
+// The same half-sweep marching in z (large levels): a thread owns position
+// (m, j) of a plane, both halves, and K consecutive planes.  In plane k the
+// colour's cells are in half h = (parity + k) & 1, so the thread updates one
+// cell per plane, i = 2m + ((h + j) & 1).  Its z-neighbours are the same
+// half of the planes k-1 and k+1 (the other colour), so the one above plane
+// k is the one below plane k+2 and is loaded once.  Every load of the slab
+// (the other colour only, which the sweep does not write) is issued before
+// the first store, the coefficients of the thread's two columns and row once
+// per thread, those of z (wave-uniform) from the scalar cache.  The update is
+// rb_point_mjk's, in its order.
+template <class T, int K>
+__global__ __launch_bounds__(kBlock) void k_rb_march(const DevLevel* __restrict__ Lt, LaunchList ll,
+                                                     int parity, int zero_guess) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevLevel& L = Lt[ll.items[t]];
+  if (level_off(L)) return;
+  const int zg = zero_flags(L, zero_guess);
+  const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
+  const int npos = hx * ny, ncb = (npos + kBlock - 1) / kBlock;
+  const int lb = b - ll.prefix[t];
+  const int bk = lb / ncb, pos = (lb - bk * ncb) * kBlock + threadIdx.x;
+  if (pos >= npos) return;
+  const int j = gm_div(pos, L.gmhx, L.gshx), m = pos - j * hx;
+  const int k0 = bk * K;
+  const sp<kG, const T> co = coefs<T>(L);
+  const sp<kG, const T> cy = co + 3 * nx;
+  const sp<kG, const T> cz = cy + 3 * ny;
+  const auto u = gfield<T>(L.u);
+  const auto f = gfield<const T>((const void*)L.f);
+  // the thread's two columns (half h: i = 2m + ((h + j) & 1))
+  int ml[2], mr[2];
+  T wxl[2], wxr[2], dxx[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = 2 * m + ((h + j) & 1), r = i & 1;
+    ml[h] = m - 1 + r; if (ml[h] < 0) ml[h] = hx - 1;
+    mr[h] = m + r;     if (mr[h] >= hx) mr[h] = 0;
+    wxl[h] = co[i];
+    wxr[h] = co[nx + i];
+    dxx[h] = co[2 * nx + i];
+  }
+  const int jm = (j == 0) ? ny - 1 : j - 1, jp = (j == ny - 1) ? 0 : j + 1;
+  const T wys = cy[j], wyn = cy[ny + j], dyy = cy[2 * ny + j];
+  const bool nbr = !(zg & 1);
+  // plane k of half h, row jj: index of position mm
+  auto at = [&](int k, int h, int jj, int mm) { return ((k * ny + jj) * 2 + h) * hx + mm; };
+  T fv[K], xl[K], xr[K], ys[K], yn[K], zp[K], zm01[2];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int k = min(k0 + s, nz - 1);              // a short last slab re-reads its last plane
+    const int h = (parity + k0 + s) & 1, o = h ^ 1;
+    fv[s] = f[at(k, h, j, m)];
+    if (nbr) {
+      xl[s] = u[at(k, o, j, ml[h])];
+      xr[s] = u[at(k, o, j, mr[h])];
+      ys[s] = u[at(k, o, jm, m)];
+      yn[s] = u[at(k, o, jp, m)];
+      zp[s] = u[at(k == nz - 1 ? 0 : k + 1, h, j, m)];
+    }
+  }
+  if (nbr) {
+    // below the slab's first two planes
+    const int h0 = (parity + k0) & 1;
+    zm01[0] = u[at(k0 == 0 ? nz - 1 : k0 - 1, h0, j, m)];
+    zm01[1] = u[at(k0, h0 ^ 1, j, m)];
+  }
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int k = k0 + s;
+    if (k >= nz) break;
+    const int h = (parity + k) & 1;
+    const T d = (cz[2 * nz + k] + dyy) + dxx[h];
+    T val;
+    if (!nbr) {
+      val = over(fv[s], d);
+    } else {
+      const T zmv = s < 2 ? zm01[s] : zp[s - 2];
+      T a = wxl[h] * xl[s] + wxr[h] * xr[s];
+      a += wys * ys[s];
+      a += wyn * yn[s];
+      a += cz[k] * zmv;
+      a += cz[nz + k] * zp[s];
+      val = over(fv[s] - a, d);
+    }
+    u[at(k, h, j, m)] = val;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // z-line red-black Gauss-Seidel half-sweep (mgMK/mg_relax.F90:423-600, LJ):
 // the columns (i,j) of one colour ((i+j)&1 == parity) solve their
@@ -2190,6 +2279,16 @@ static int device_cus() {
   return cus[dev];
 }
 
+void launch_rb_march(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
+                     int planes, hipStream_t s) {
+#define RM(T, K) LAUNCH((k_rb_march<T, K>), ll.nblocks, kBlock, L, ll, parity, zero_guess)
+  if (fp64) {
+    if (planes == 8) RM(double, 8); else if (planes == 4) RM(double, 4); else RM(double, 2);
+  } else {
+    if (planes == 8) RM(float, 8); else if (planes == 4) RM(float, 4); else RM(float, 2);
+  }
+#undef RM
+}
 void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                      hipStream_t s) {
   if (fp64) LAUNCH(k_rb_point<double>, ll.nblocks, kBlock, L, ll, parity, zero_guess);

@@ -2,11 +2,10 @@
 //
 // A worker thread owns one stream and advances every in-flight request by
 // one V-cycle per round, with one launch per level step for all of them
-// (Context::cycle_and_norms); a round whose signature (its requests) came
-// before runs as a captured graph.  Each request has
-// its own gate on the device (ItemGate): the round's last norm fold applies
-// the stop rule to every request, and a request that stops has its kernels
-// in later rounds return at once.  After a round, one copy brings every
+// (Context::cycle_and_norms).  Each request has its own gate on the device
+// (ItemGate): the round's last norm fold applies the stop rule to every
+// request, and a request that stops has its kernels in later rounds return
+// at once.  After a round, one copy brings every
 // gate into a pinned mirror, followed by an event; the host waits for that
 // event (not for the stream), replays the gates (Context::absorb_gate) and
 // retires the finished requests (null-space removal, ghosts and the copy
@@ -71,32 +70,14 @@ struct Request {
 // rounds in flight were measured 5 and 10 % slower in solves/s at 1000
 // requests/s on the served mix, with longer latencies.
 constexpr int kPipe = 1;
-// Batched rounds captured as graphs (launch lists keyed by the round's
-// instances), kept for the most recent signatures.
-constexpr int kRoundGraphs = 16;
+// Rounds are launched directly, not captured as graphs: a capture in the
+// worker thread makes a device-wide synchronisation in any other thread of
+// the process fail (hipErrorStreamCaptureUnsupported, whatever the capture
+// mode), and the caller cannot know when a round is being captured.
 
 struct amg_server_s {
   Context ctx;
   ItemGate* h_mirror = nullptr;      // kPipe x kMaxInstances, pinned
-  // round graphs: signature (precision, then the instance ids) ->
-  // executable; a signature is captured when it comes back (signatures
-  // seen once are kept in 'seen')
-  struct RoundGraph {
-    std::vector<int> key;
-    hipGraphExec_t exec;
-    long last;
-  };
-  std::vector<RoundGraph> graphs;
-  std::deque<std::vector<int>> seen;
-  long tick = 0;
-  bool graphs_ok = true;
-  hipStream_t cap = nullptr;         // capture stream (never runs work)
-  void drop_round_graphs() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
-    graphs.clear();
-    seen.clear();
-  }
-  int round(const std::vector<Instance*>& C);
   double budget = 0.0;
   size_t used = 0;
   int max_slots = 1, max_batch = 1;
@@ -141,67 +122,14 @@ int amg_server_s::bind(Member& m) {
       if (x.inst && !x.busy && (!victim || x.last_use < victim->last_use)) victim = &x;
     if (!victim) return AMG_EBUSY;
     used -= victim->inst->device_bytes();
-    drop_round_graphs();
     ctx.remove_instance(victim->inst);
     victim->inst = nullptr;
   }
   Instance* I = nullptr;
-  drop_round_graphs();                 // descriptor indices and tables change
   const int rc = ctx.add_instance(m.grid, &I);
   if (rc != AMG_OK) return rc;
   used += I->device_bytes();
   m.inst = I;
-  return AMG_OK;
-}
-
-// One batched gated cycle of the instances C: from a captured graph when
-// the signature has been seen before, else launched directly.
-int amg_server_s::round(const std::vector<Instance*>& C) {
-  std::vector<int> key{C[0]->fp64 ? 1 : 0};
-  for (const Instance* I : C) {
-    key.push_back(I->id);
-  }
-  ++tick;
-  for (auto& g : graphs)
-    if (g.key == key) {
-      g.last = tick;
-      AMG_HIP(hipGraphLaunch(g.exec, ctx.stream));
-      return AMG_OK;
-    }
-  const bool again = std::find(seen.begin(), seen.end(), key) != seen.end();
-  if (!again || !graphs_ok) {
-    if (!again) {
-      seen.push_back(key);
-      if (seen.size() > 4 * kRoundGraphs) seen.pop_front();
-    }
-    return ctx.cycle_and_norms(C);
-  }
-  // capture on a stream of its own (no work of this one's is pending there)
-  hipStream_t run = ctx.stream;
-  ctx.stream = cap;
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ex = nullptr;
-  bool ok = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
-  if (ok) {
-    ok = ctx.cycle_and_norms(C) == AMG_OK;
-    ok = hipStreamEndCapture(cap, &g) == hipSuccess && ok && g;
-  }
-  ctx.stream = run;
-  if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
-  if (g) (void)hipGraphDestroy(g);
-  if (!ok) {
-    (void)hipGetLastError();
-    graphs_ok = false;
-    return ctx.cycle_and_norms(C);
-  }
-  if ((int)graphs.size() >= kRoundGraphs) {
-    auto lru = std::min_element(graphs.begin(), graphs.end(),
-                                [](const RoundGraph& a, const RoundGraph& b) { return a.last < b.last; });
-    (void)hipGraphExecDestroy(lru->exec);
-    graphs.erase(lru);
-  }
-  graphs.push_back({key, ex, tick});
-  AMG_HIP(hipGraphLaunch(ex, ctx.stream));
   return AMG_OK;
 }
 
@@ -377,7 +305,7 @@ void amg_server_s::run() {
         C.push_back(x.inst);
         rd.rids.push_back(rid);
       }
-      rc = round(C);
+      rc = ctx.cycle_and_norms(C);
       if (rc == AMG_OK &&
           hipMemcpyAsync(h_mirror + (size_t)rd.mirror * kMaxInstances, ctx.d_igate,
                          sizeof(ItemGate) * nslot, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
@@ -438,13 +366,6 @@ int amg_server_create(const amg_config* cfg, double hbm_budget_bytes, int max_sl
     set_error("amg_server_create: pinned allocation failed");
     return AMG_ENOMEM;
   }
-  if (hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking) != hipSuccess) {
-    s->cap = nullptr;
-    (void)hipHostFree(s->h_mirror);
-    delete s;
-    set_error("amg_server_create: stream creation failed");
-    return AMG_EHIP;
-  }
   s->budget = hbm_budget_bytes;
   s->max_slots = max_slots;
   s->max_batch = std::min(max_batch, kMaxItems);
@@ -465,8 +386,6 @@ int amg_server_destroy(amg_server s) {
   for (auto& m : s->members)
     if (m.d_warm) (void)hipFree(m.d_warm);
   if (s->h_mirror) (void)hipHostFree(s->h_mirror);
-  s->drop_round_graphs();
-  if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
   return AMG_OK;
 }

@@ -608,6 +608,26 @@ void Context::zline_launches(const std::vector<Item>& items, bool fp64, int pari
       });
 }
 
+// Point half-sweeps of the items' levels: the levels large enough march in z
+// (rb_march_planes), bucketed by slab length; the others take one thread
+// per cell.  The items' block counts are recomputed here.
+void Context::rb_launches(const std::vector<Item>& items, bool fp64, int parity, int zg) {
+  std::vector<Item> bucket[9];
+  for (const Item& it : items) {
+    const DevLevel& d = hL[it.idx];
+    const int K = rb_march_planes(d.hx, d.ny, d.nz);
+    const int blocks = K ? cdiv((long long)d.hx * d.ny, kBlock) * ((d.nz + K - 1) / K)
+                         : cdiv((long long)d.hx * d.ny * d.nz, kBlock);
+    bucket[K].push_back({it.idx, blocks});
+  }
+  for (int K : {0, 2, 4, 8})
+    if (!bucket[K].empty())
+      for_lists(bucket[K], [&](const LaunchList& ll, int, int) {
+        if (K) launch_rb_march(fp64, dL, ll, parity, zg, K, stream);
+        else launch_rb_point(fp64, dL, ll, parity, zg, stream);
+      });
+}
+
 // Direct solves: the plane and mode lists hold the same levels in the same
 // order, launched in chunks of at most kMaxItems levels.
 void Context::direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes,
@@ -638,9 +658,7 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
   const DevLevel* L = dL;
   auto half = [&](const std::vector<Item>& items, int parity, int zg) {
     if (zline) zline_launches(items, fp64, parity, zg);
-    else for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_rb_point(fp64, L, ll, parity, zg, stream);
-    });
+    else rb_launches(items, fp64, parity, zg);
   };
   auto blocks_smooth = [&](const DevLevel& d) {
     return zline ? cdiv((long long)d.hx * d.ny, 64) : cdiv((long long)d.hx * d.ny * d.nz, kBlock);
@@ -1229,7 +1247,7 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
       if (which == 0 || which == 4 || which == 6) {
         const int par = which == 4 ? 1 : 0, zg = which == 6 ? 3 : 0;
         if (zline) zline_launches(items, fp64, par, zg, form);
-        else launch_rb_point(fp64, dL, ll, par, zg, stream);
+        else rb_launches(items, fp64, par, zg);
       } else if (which == 7 || which == 8) {
         launch_zline_col(fp64, dL, ll, which == 8 ? 1 : 0, 0, stream);
       } else if (which == 1) {

@@ -120,6 +120,7 @@ struct Context {
   // pieces
   void vcycle(const std::vector<Instance*>& S, bool fp64);
   void zline_launches(const std::vector<Item>& items, bool fp64, int parity, int zg, int form = 0);
+  void rb_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
   void direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes, size_t lds,
                        bool fp64);
   int prepare(const std::vector<Instance*>& S);

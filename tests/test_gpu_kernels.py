@@ -85,6 +85,41 @@ def test_half_sweeps(cuda, case, smoother, precision):
                 assert relerr(got, ref) < _tol(precision, smoother), (lvl, which)
 
 
+@pytest.mark.parametrize("n,beta,tbc,nlevels", [
+    # slabs of 8 planes: the production grid's finest level (Dirichlet z walls)
+    ((256, 64, 256), 0.0, (0, 2, 0, 2, 0, 0), None),
+    # slabs of 4 planes, the last one of 2, periodic in every direction
+    ((512, 128, 42), 1.0, (0, 0, 0, 0, 0, 0), (7, 7, 1)),
+])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_point_sweep_march_large_level(cuda, precision, n, beta, tbc, nlevels):
+    """The point half-sweep on a level large enough to march in z (a thread
+    per chequer position over a slab of planes, the z-neighbour above plane
+    k reused below plane k+2) against the oracle's point sweep, both
+    colours, from a guess and from zero."""
+    from oracle import vcycle as VC
+    s, g, levs, fac = _solver(n, beta, tbc, "RBPGS", precision, nlevels=nlevels)
+    dt = np.float64 if precision == "fp64" else np.float32
+    rng = np.random.default_rng(77)
+    lev = levs[0]
+    nx, ny, nz = lev["n"]
+    u = rng.standard_normal((nz, ny, nx)).astype(dt)
+    f = rng.standard_normal((nz, ny, nx)).astype(dt)
+    for which, parity in ((0, 0), (4, 1), (6, 0)):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        s.run_kernel(which, 0, 1)
+        got = s.level_field(0, 0)
+        ref = np.zeros_like(u) if which == 6 else u.copy()
+        VC.rb_half_sweep(ref, f, lev, parity, dt)
+        if which == 6:
+            m = VC.colour_mask(u.shape, parity, False)
+            assert relerr(got[m], ref[m]) < _tol(precision, "RBPGS"), which
+        else:
+            assert relerr(got, ref) < _tol(precision, "RBPGS"), which
+    s.close()
+
+
 @pytest.mark.parametrize("tbc", [(0, 1, 0, 1, 0, 0), (2, 2, 1, 1, 2, 2), (0, 0, 0, 0, 0, 0)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_restriction_march_large_level(cuda, precision, tbc):

@@ -339,3 +339,27 @@ def test_paired_cycles(cuda, smoother):
           f"({info['cycles']})")
     # FP32 rounding differs between numpy and the device: the stop may move by one cycle
     assert abs(info["iters"] - its[1]) <= 1 and len(info["cycles"]) < info["iters"]
+
+
+def test_server_allows_device_synchronisation(cuda):
+    """The caller's thread may synchronise the whole device while the
+    server's worker runs its rounds (the worker captures no graphs, which
+    would make such a call fail); every request completes."""
+    torch = cuda
+    from anelastic_solve_amd import Server, box_grid, default_config, workloads
+    specs = [((32, 32, 32), 1.0), ((16, 16, 16), 2.0), ((48, 48, 24), 1.5)]
+    grids = [box_grid(*n, beta=b) for n, b in specs]
+    srv = Server(default_config(), hbm_budget=2e9, max_slots=3, max_batch=3)
+    mids = [srv.add_member(g) for g in grids]
+    rhs = [torch.from_numpy(workloads.psource_host(*workloads.velocity_white(g, 5 + q), g)).cuda()
+           for q, g in enumerate(grids)]
+    outs = [torch.zeros(g.shape, dtype=torch.float64, device="cuda") for g in grids]
+    reqs = []
+    for rep in range(10):
+        for q in range(3):
+            reqs.append(srv.submit(mids[q], rhs[q], outs[q]))
+        for _ in range(20):
+            torch.cuda.synchronize()
+    srv.drain()
+    assert all(srv.poll(r)[0] == 0 for r in reqs)
+    srv.close()
