@@ -41,6 +41,7 @@ struct GhostPlan {
 // side (kind 1) or the physics row's viscous factors c1, m1, c2, m2, m3.
 struct CnRow {
   double ap, bp, c;
+  double rbp, cbp;  // 1/bp and -c/bp (the segmented solve's products)
   double cval;
   double c1, m1, c2, m2, m3;
   double pz;        // 1/dze(k): the predicted-pressure metric of w

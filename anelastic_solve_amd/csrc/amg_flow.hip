@@ -1,12 +1,14 @@
 // Time-stepper boundary handling and the implicit vertical viscous step:
 //   ghost(var, vtype)            dns/BC.F90:1-187 (+ bound_update :334-584), serial
-//   var_advance_CN_PL(...)       dns/NSsolver.F90:3054-3465, DNS branch, serial Thomas
-//                                (mgMK/mg_relax.F90:541-595 on every interior column)
+//   var_advance_CN_PL(...)       dns/NSsolver.F90:3054-3465, DNS branch: the Thomas
+//                                solve of mgMK/mg_relax.F90:541-595 on every interior
+//                                column, in segments (nz <= 256) or serially
 // Fields are the reference's ghosted FP64 arrays, x fastest:
 //   e(i,j,k) = (k*(ny+2) + j)*(nx+2) + i,   0 <= i <= nx+1 (ghosts 0 and nx+1).
 // Floating-point contraction is off in this file: every product and sum is
 // rounded as written, in the order of the Fortran source, so the results
-// equal a plain IEEE evaluation of the reference's expressions.
+// equal a plain IEEE evaluation of the reference's expressions (the
+// segmented line solve: to rounding).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -151,8 +153,9 @@ __global__ __launch_bounds__(kFlowBlock) void k_flow_face(double* __restrict__ a
   if (nrules > 1) face_rule(a, base, st, n, r2, 2, d);
 }
 
-// var_advance_CN_PL on one interior column (i, j): the right-hand side of
-// every physics row from the old values (f1*v2 + f2*v1*f3, the predicted
+// var_advance_CN_PL on one interior column (i, j) per lane (nz > 256;
+// k_cn_seg below takes nz <= 256): the right-hand side of every physics row
+// from the old values (f1*v2 + f2*v1*f3, the predicted
 // pressure term when p is given, the explicit half of the vertical viscous
 // term), the constant rows (walls, and rows a ghost rule zeroed), then the
 // mgMK Thomas with the column-independent eliminated rows ap/bp: forward
@@ -268,6 +271,169 @@ __global__ __launch_bounds__(kFlowBlock) void k_cn_column(const double* __restri
       v2[base + (size_t)kk * st] = x;
     }
   }
+}
+
+// The same step with every column split into P segments of S interior rows
+// (at most kCnSeg rows and kCnSeg segments: nz <= 256), one wave per segment
+// and 64 columns per workgroup.  The recurrences' coefficients are
+// the same in every column, so their values are linear in the value carried
+// into a segment:
+//   forward   G_k = G'_k + (prod_{i=r0..k} ap_i) G_{r0-1},        G' from G_{r0-1} = 0,
+//   backward  x_k = x'_k + (prod_{i=k..r1-1} (-c_i/bp_i)) x_{r1},  x' from x_{r1} = 0.
+// Each wave solves its segment locally with the serial kernel's operations
+// (the division by the pivot as a product with its reciprocal, from the
+// table), the segments' end values are chained through LDS (a scan over at
+// most 16 segments), and each row adds its carry term: a few roundings more
+// per row than the serial solve, whose result this equals to rounding.
+// The column is read once and written once (24 B/cell against the serial
+// kernel's 40), and a 256^2 grid has 16 waves per column group instead of
+// one per SIMD.  Rows 0 and nz+1 (the wall rows) are taken by the first and
+// the last wave with the serial kernel's expressions.  Every load comes
+// before the first barrier and every store after the last, so the column is
+// rewritten in place.
+constexpr int kCnSeg = 16;     // rows per wave and waves per column group, at most
+constexpr int kCnChunk = 4;    // rows whose loads are in flight together
+template <int PV>
+__global__ __launch_bounds__(1024) void k_cn_seg(const double* __restrict__ v1,
+                                                 double* __restrict__ v2,
+                                                 const double* __restrict__ p,
+                                                 const CnRow* __restrict__ T,
+                                                 const double* __restrict__ rdxe,
+                                                 const double* __restrict__ rdye, int nx, int ny,
+                                                 int nz, int S, double f1, double f2, double f3,
+                                                 double rkdt, double h) {
+  __shared__ double gend[kCnSeg][64], xfirst[kCnSeg][64], g0[64], xtop[64];
+  __shared__ double pend[kCnSeg], qfirst[kCnSeg];
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y), P = blockDim.y;
+  const int t = blockIdx.x * 64 + lane;
+  const bool live = t < nx * ny;
+  const int tc = live ? t : nx * ny - 1;              // a spare lane repeats the last column
+  const int i = 1 + tc % nx, j = 1 + tc / nx;
+  const size_t st = (size_t)(nx + 2) * (ny + 2);
+  const size_t base = (size_t)j * (nx + 2) + i;
+  const int n = nz + 2;
+  constexpr bool pcv = PV != 0;
+  const size_t poff = PV == 1 ? 1 : (size_t)(nx + 2);
+  auto at = [&](int k) { return base + (size_t)k * st; };
+  auto pnb = [&](int k) -> size_t {
+    return PV == 3 ? at(min(k + 1, n - 1)) : at(k) + poff;
+  };
+  const double dmc = PV == 1 ? rdxe[i] : (PV == 2 ? rdye[j] : 0.0);
+  // the serial kernel's right-hand side of row k
+  auto rhs = [&](int k, double vm, double vc, double vp, double a1, double pa, double pb) {
+    const CnRow R = T[k];
+    double r = f1 * vc + f2 * a1 * f3;
+    if constexpr (pcv) r = r - rkdt * (pb - pa) * (PV == 3 ? R.pz : dmc);
+    const double tv = R.c1 * (vp - vc) * R.m1 - R.c2 * (vc - vm) * R.m2;
+    r = r + h * tv * R.m3;
+    return R.kind ? R.cval : r;
+  };
+  const int r0 = 1 + w * S, r1 = min(r0 + S, nz + 1);
+  // the wall rows' inputs, with the segment's (all before the first barrier)
+  double G0 = 0.0, rtop = 0.0;
+  if (w == 0)
+    G0 = rhs(0, 0.0, v2[at(0)], v2[at(1)], v1[at(0)], pcv ? p[at(0)] : 0.0,
+             pcv ? p[pnb(0)] : 0.0);
+  if (w == P - 1)
+    rtop = rhs(n - 1, v2[at(n - 2)], v2[at(n - 1)], v2[at(n - 1)], v1[at(n - 1)],
+               pcv ? p[at(n - 1)] : 0.0, pcv ? p[pnb(n - 1)] : 0.0);
+  // forward, locally: the loads of kCnChunk rows, then their steps
+  double G[kCnSeg];
+  double g = 0.0, pr = 1.0;
+#pragma unroll
+  for (int c0 = 0; c0 < kCnSeg; c0 += kCnChunk) {
+    if (c0 < S) {
+      double wv[kCnChunk + 2], a1[kCnChunk], pa[kCnChunk], pb[kCnChunk];
+#pragma unroll
+      for (int q = 0; q < kCnChunk + 2; ++q) wv[q] = v2[at(min(r0 - 1 + c0 + q, n - 1))];
+#pragma unroll
+      for (int q = 0; q < kCnChunk; ++q) {
+        const int k = min(r0 + c0 + q, n - 1);
+        a1[q] = v1[at(k)];
+        pa[q] = pcv ? p[at(k)] : 0.0;
+        pb[q] = pcv ? p[pnb(k)] : 0.0;
+      }
+      // the right-hand sides (independent of the recurrence), then its steps
+      double rr[kCnChunk], apq[kCnChunk];
+#pragma unroll
+      for (int q = 0; q < kCnChunk; ++q) {
+        const int k = min(r0 + c0 + q, n - 1);
+        rr[q] = rhs(k, wv[q], wv[q + 1], wv[q + 2], a1[q], pa[q], pb[q]);
+        apq[q] = T[k].ap;
+      }
+#pragma unroll
+      for (int q = 0; q < kCnChunk; ++q) {
+        if (r0 + c0 + q < r1) {
+          g = rr[q] + apq[q] * g;
+          pr = apq[q] * pr;
+        }
+        G[c0 + q] = g;
+      }
+    }
+  }
+  gend[w][lane] = g;
+  if (w == 0) g0[lane] = G0;
+  if (lane == 0) pend[w] = pr;
+  __syncthreads();
+  // G_{r0-1}, the value carried into this segment, and the segment's G
+  double C = g0[lane];
+  for (int q = 0; q < w; ++q) C = gend[q][lane] + pend[q] * C;
+  pr = 1.0;
+#pragma unroll
+  for (int q = 0; q < kCnSeg; ++q) {
+    const int k = min(r0 + q, n - 1);
+    const double ap = T[k].ap;
+    if (r0 + q < r1) {
+      pr = ap * pr;
+      G[q] = G[q] + pr * C;
+    }
+  }
+  // the top wall row: G_{nz+1} from G_nz, x_{nz+1}
+  if (w == P - 1) {
+    double Gn = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCnSeg; ++q)
+      if (r0 + q == r1 - 1) Gn = G[q];
+    xtop[lane] = (rtop + T[n - 1].ap * Gn) * T[n - 1].rbp;
+  }
+  // backward, locally (the rows' coefficients read before the chain)
+  double cq[kCnSeg], rq[kCnSeg], bq[kCnSeg];
+#pragma unroll
+  for (int q = 0; q < kCnSeg; ++q) {
+    const int k = min(r0 + q, n - 1);
+    cq[q] = T[k].c;
+    rq[q] = T[k].rbp;
+    bq[q] = T[k].cbp;
+  }
+  double x = 0.0, qr = 1.0;
+#pragma unroll
+  for (int q = kCnSeg - 1; q >= 0; --q) {
+    if (r0 + q < r1) {
+      x = (G[q] - cq[q] * x) * rq[q];
+      qr = bq[q] * qr;
+      G[q] = x;
+    }
+  }
+  xfirst[w][lane] = x;
+  if (lane == 0) qfirst[w] = qr;
+  __syncthreads();
+  // x_{r1}, the value carried into this segment from above, and the stores
+  double X = xtop[lane];
+  for (int q = P - 1; q > w; --q) X = xfirst[q][lane] + qfirst[q] * X;
+  double qk = 1.0, x1 = 0.0;
+#pragma unroll
+  for (int q = kCnSeg - 1; q >= 0; --q) {
+    const int k = r0 + q;
+    if (k < r1) {
+      qk = bq[q] * qk;
+      const double xk = G[q] + qk * X;
+      if (live) v2[at(k)] = xk;
+      x1 = xk;
+    }
+  }
+  if (live && w == 0) v2[at(0)] = (G0 - T[0].c * x1) * T[0].rbp;
+  if (live && w == P - 1) v2[at(n - 1)] = xtop[lane];
 }
 
 unsigned cdiv(long long a, int b) { return (unsigned)((a + b - 1) / b); }
@@ -491,6 +657,8 @@ int plan_cn(const FlowGrid& G, int var, double rkdt, double rRe, double rPr, dou
     R[k].ap = a[k];
     R[k].bp = b[k];
     R[k].c = sup[k];
+    R[k].rbp = 1.0 / b[k];
+    R[k].cbp = -(sup[k] / b[k]);
   }
   *rows = std::move(R);
   return 0;
@@ -499,9 +667,21 @@ int plan_cn(const FlowGrid& G, int var, double rkdt, double rRe, double rPr, dou
 void launch_cn_column(const double* v1, double* v2, const double* p, const CnRow* T,
                       const double* rdxe, const double* rdye, int nx, int ny, int nz, double f1,
                       double f2, double f3, double rkdt, int var, hipStream_t s) {
-  const dim3 g(cdiv((long long)nx * ny, kFlowBlock)), b(kFlowBlock);
   const int pv = (p != nullptr && var <= 3) ? var : 0;
   const double h = rkdt * 0.5;
+  if (nz <= kCnSeg * kCnSeg) {
+    // segments of S <= 16 rows, at most 16 of them
+    const int S = (nz + kCnSeg - 1) / kCnSeg, P = (nz + S - 1) / S;
+    const dim3 g(cdiv((long long)nx * ny, 64)), b(64, P);
+#define CS(V) hipLaunchKernelGGL(k_cn_seg<V>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, S, f1, f2, f3, rkdt, h)
+    if (pv == 1) CS(1);
+    else if (pv == 2) CS(2);
+    else if (pv == 3) CS(3);
+    else CS(0);
+#undef CS
+    return;
+  }
+  const dim3 g(cdiv((long long)nx * ny, kFlowBlock)), b(kFlowBlock);
   if (pv == 1) hipLaunchKernelGGL(k_cn_column<1>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
   else if (pv == 2) hipLaunchKernelGGL(k_cn_column<2>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);
   else if (pv == 3) hipLaunchKernelGGL(k_cn_column<3>, g, b, 0, s, v1, v2, p, T, rdxe, rdye, nx, ny, nz, f1, f2, f3, rkdt, h);

@@ -6,10 +6,13 @@ through the C ABI, against ``oracle.dns``.
 * ``Pgrad`` with its three ghost calls: equal to the oracle's ``pgrad``
   followed by ``ghost`` to 1e-14 of the field scale (the interior update
   is the existing Pgrad kernel, whose multiply-adds may be fused);
-* ``var_advance_CN_PL``: bit-exact for u, v, w, rho, scal1 with Neumann,
-  Dirichlet, Wall6 and Wall7 rows, with and without the predicted pressure,
-  on a stretched grid with a varying rho(z) (the CN kernels evaluate the
-  reference's expressions without contraction, as numpy does);
+* ``var_advance_CN_PL``: equal to rounding (1e-13 of the field scale) for
+  u, v, w, rho, scal1 with Neumann, Dirichlet, Wall6 and Wall7 rows, with
+  and without the predicted pressure, on a stretched grid with a varying
+  rho(z): the kernel evaluates the reference's expressions without
+  contraction, as numpy does, and splits each column's Thomas recurrences
+  into segments joined by carried values (a few more roundings per row;
+  nz = 264 takes the serial kernel);
 * a substep tail on the device (CN for u, v, w, then Psource, the solve and
   Pgrad with ghosts) ends divergence-free.
 """
@@ -35,7 +38,9 @@ def _axes(g):
 
 def _solver(g):
     from anelastic_solve_amd import PressureSolver, default_config
-    return PressureSolver(g, default_config(precision="fp64", tol=1e-12))
+    # the z-line smoother takes nz <= 256; the flow steps do not depend on it
+    sm = "LJ" if g.nz <= 256 else "RBPGS"
+    return PressureSolver(g, default_config(precision="fp64", tol=1e-12, smoother=sm))
 
 
 def _tables():
@@ -114,11 +119,12 @@ CN_CASES = [
 
 
 @pytest.mark.parametrize("var,walls,pcorrect", CN_CASES)
-def test_cn_bit_exact(cuda, var, walls, pcorrect):
+@pytest.mark.parametrize("nz", [20, 256, 264])
+def test_cn_step(cuda, var, walls, pcorrect, nz):
     torch = cuda
     from anelastic_solve_amd import FlowBC
     from oracle import dns
-    g = _grid()
+    g = _grid(nx=32, ny=24 if nz == 20 else 32, nz=nz)
     axes = _axes(g)
     s = _solver(g)
     bc = FlowBC.make((True, True, False), walls)
@@ -132,7 +138,9 @@ def test_cn_bit_exact(cuda, var, walls, pcorrect):
     s.var_advance_CN_PL(d1, d2, args["f1"], args["f2"], args["f3"], args["rkdt"], var, bc,
                         args["rRe"], args["rPr"], args["rSc"], p=dp if pcorrect else None)
     got = d2.cpu().numpy()
-    assert np.array_equal(got, want), float(np.abs(got - want).max())
+    err = float(np.abs(got - want).max() / np.abs(want).max())
+    print(f"var {var} nz {nz}: {err:.2e}")
+    assert err <= 1e-13, err
     assert torch.equal(d1, torch.from_numpy(v1).cuda())      # veltemp1 is read only
     s.close()
 

@@ -9,7 +9,7 @@ reference reader's value slips and case-sensitive type names; the density
 is a decreasing moist-air profile whose constants carry their float32 values.
 GPU: the production solve (Dirichlet pressure on the z walls, rho(z),
 4/3/8 levels per direction, RBPGS with the run's per-level sweeps) against
-the oracle's direct solve, and a CN step with the run's BCs, bit-exact.
+the oracle's direct solve, and a CN step with the run's BCs to rounding.
 """
 import math
 import os
@@ -138,7 +138,9 @@ def test_production_solve_against_oracle(cuda):
 
 
 @pytest.mark.gpu
-def test_production_cn_step_bit_exact(cuda):
+def test_production_cn_step(cuda):
+    """The vertical CN step with the run's BCs equals the oracle to rounding
+    (1e-13 of the field scale; the columns' recurrences run in segments)."""
     torch = cuda
     from anelastic_solve_amd import PressureSolver, workloads
     from oracle import dns
@@ -154,5 +156,7 @@ def test_production_cn_step_bit_exact(cuda):
         d2 = torch.from_numpy(field.copy()).cuda()
         s.var_advance_CN_PL(torch.from_numpy(tend).cuda(), d2, 1.0, -5.0 / 9.0, 1.0, 0.5, var, bc,
                             1.0 / 5000.0, 1.0 / 0.7, 1.0)
-        assert np.array_equal(d2.cpu().numpy(), want), var
+        got = d2.cpu().numpy()
+        err = float(np.abs(got - want).max() / np.abs(want).max())
+        assert err <= 1e-13, (var, err)
     s.close()
