@@ -1562,16 +1562,22 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
   block_reduce_store<NV, false>(v, partial + (size_t)b * NV);
 }
 
-// Writes the state words of instance inst on all its levels (a negative
+// State words an instance's levels take at the end of a cycle (a negative
 // value leaves that word as it is; pflip toggles pcur).
-__device__ void set_words(DevLevel* Lt, int nL, int inst, int stop, int warm, int skipc, int pflip) {
-  for (int l = 0; l < nL; ++l) {
+struct GateWords {
+  int stop, warm, skipc, pflip;
+};
+
+// Writes w on the levels of instance inst among entries l0, l0+step, ...
+// of the table (the threads of a block split the table between them).
+__device__ void set_words(DevLevel* Lt, int nL, int inst, GateWords w, int l0, int step) {
+  for (int l = l0; l < nL; l += step) {
     DevLevel& d = Lt[l];
     if (d.inst != inst) continue;
-    if (stop >= 0) d.stop = stop;
-    if (warm >= 0) d.warm = warm;
-    if (skipc >= 0) d.skipc = skipc;
-    if (pflip) d.pcur ^= 1;
+    if (w.stop >= 0) d.stop = w.stop;
+    if (w.warm >= 0) d.warm = w.warm;
+    if (w.skipc >= 0) d.skipc = w.skipc;
+    if (w.pflip) d.pcur ^= 1;
   }
 }
 
@@ -1580,41 +1586,46 @@ __device__ void set_words(DevLevel* Lt, int nL, int inst, int stop, int warm, in
 // second; otherwise the correction's norms are recorded and the stop rule
 // of the host's norm step (Context::post_norms) applied to them with the
 // same double operations, so the host's replay reaches the same decision;
-// then the next cycle is planned (paired or not).
-__device__ void item_gate_check(ItemGate* g, const double* out, const double* outc, DevLevel* Lt,
-                                int nL, const DevLevel& L) {
-  if (g->stop) return;
-  const int inst = L.inst;
-  g->cycles += 1;
-  if (L.skipc) {
-    set_words(Lt, nL, inst, -1, 1, 0, 0);
-    return;
-  }
-  const int m = L.warm ? 2 : 1;
-  const int n = g->count;
+// then the next cycle is planned (paired or not).  Returns the state words
+// the instance's levels take.
+__device__ GateWords item_gate_check(ItemGate* g, const double* out, const double* outc,
+                                     const DevLevel& L) {
+  // every load first (the stores below may alias them as far as the
+  // compiler knows, which would serialise a round trip per load)
+  double o[4], oc[2];
+  for (int q = 0; q < 4; ++q) o[q] = out[q];
+  oc[0] = outc[0];
+  oc[1] = outc[1];
+  const int gstop = g->stop, cycles = g->cycles + 1, n = g->count, maxcy = g->maxcy;
+  const int rule = g->rule, fp64 = g->fp64, report = g->report, gpair = g->pair;
+  const double bnorm = g->bnorm, tol = g->tol, rprev = g->rprev;
+  const int skipc = L.skipc, warm = L.warm;
+  if (gstop) return {-1, -1, -1, 0};
+  g->cycles = cycles;
+  if (skipc) return {-1, 1, 0, 0};
+  const int m = warm ? 2 : 1;
   double* h = g->hist[n % kItemRing];
-  for (int q = 0; q < 4; ++q) h[q] = out[q];
-  h[4] = g->report ? outc[0] : 0.0;
-  h[5] = g->report ? outc[1] : 0.0;
+  for (int q = 0; q < 4; ++q) h[q] = o[q];
+  h[4] = report ? oc[0] : 0.0;
+  h[5] = report ? oc[1] : 0.0;
   g->hcyc[n % kItemRing] = m;
-  const double rmax = out[0];
+  const double rmax = o[0];
   double relmax;
-  if (g->fp64) relmax = g->report ? (h[5] > 0.0 ? h[4] / h[5] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
-  else relmax = out[2] > 0.0 ? out[1] / out[2] : 0.0;
-  const double relres = g->bnorm > 0.0 ? rmax / g->bnorm : rmax;
+  if (fp64) relmax = report ? (oc[1] > 0.0 ? oc[0] / oc[1] : 0.0) : __longlong_as_double(0x7ff8000000000000ll);
+  else relmax = o[2] > 0.0 ? o[1] / o[2] : 0.0;
+  const double relres = bnorm > 0.0 ? rmax / bnorm : rmax;
   g->count = n + 1;
   int stop = 0;
   if (!isfinite(rmax)) {
     stop = 1;
   } else {
-    const double crit = (g->rule == AMG_STOP_RELMAX) ? relmax : relres;
-    if (crit <= g->tol || g->cycles >= g->maxcy) stop = 1;
+    const double crit = (rule == AMG_STOP_RELMAX) ? relmax : relres;
+    if (crit <= tol || cycles >= maxcy) stop = 1;
   }
-  const bool pair = !stop && g->pair && g->cycles + 2 <= g->maxcy &&
-                    pair_next(relres, g->rprev, m, g->tol);
+  const bool pair = !stop && gpair && cycles + 2 <= maxcy && pair_next(relres, rprev, m, tol);
   g->rprev = relres;
   g->stop = stop;
-  set_words(Lt, nL, inst, stop, 0, pair ? 1 : 0, g->fp64 ? 0 : 1);
+  return {stop, 0, pair ? 1 : 0, fp64 ? 0 : 1};
 }
 
 // One thread per item: the gate of a starting solve, from its first defect
@@ -1644,13 +1655,13 @@ __global__ void k_gate_init(LaunchList ll, const double* __restrict__ out, ItemG
   else if (rule == AMG_STOP_RESIDUAL && (b == 0.0 || g->res0 <= tol)) stop = 1;
   g->stop = stop;
   const bool pair = !stop && g->pair && maxcy >= 2 && g->res0 * kPairRho0 > tol;
-  set_words(Lt, nL, L.inst, stop, 0, pair ? 1 : 0, 0);
+  set_words(Lt, nL, L.inst, {stop, 0, pair ? 1 : 0, 0}, 0, 1);
 }
 
 // One block per item: thread t folds partials b0+t, b0+t+kFinBlock, ... in
 // order, then a fixed-shape tree combines the threads (deterministic).  With
-// a gate (a gated cycle's last fold, one item), thread 0 then ends the cycle;
-// with per-instance gates, thread 0 of every item's block ends its cycle.
+// per-instance gates, thread 0 of every item's block then ends its cycle and
+// the block's threads write the resulting state words, one table entry each.
 constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const double* __restrict__ partial,
                                                         int nvals, int op_sum, double* __restrict__ out,
@@ -1693,10 +1704,16 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
     }
     __syncthreads();
   }
-  if (ga.ig && threadIdx.x == 0) {
-    const DevLevel& L = ga.Lt[ll.items[t]];
-    item_gate_check(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, ga.Lt, ga.nL,
-                    L);
+  if (ga.ig) {
+    __shared__ GateWords gw;
+    __shared__ int ginst;
+    if (threadIdx.x == 0) {
+      const DevLevel& L = ga.Lt[ll.items[t]];
+      gw = item_gate_check(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, L);
+      ginst = L.inst;
+    }
+    __syncthreads();
+    set_words(ga.Lt, ga.nL, ginst, gw, threadIdx.x, kFinBlock);
   }
 }
 

@@ -192,6 +192,37 @@ def test_transfers_and_coarse_solve(cuda, case, precision):
     assert relerr(got, ref) < (1e-11 if precision == "fp64" else 2e-6)
 
 
+@pytest.mark.parametrize("tbc", [(0, 1, 0, 1, 0, 0), (0, 0, 0, 0, 0, 0)])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_zline_ragged_full_segments(cuda, precision, tbc):
+    """A level with more column groups than the device has CUs (128 x 128
+    columns per colour: the lean form in FP32) and ragged columns of 40
+    cells (two full 16-cell segments and a tail of 8), Neumann and periodic
+    in z, both colours, from a guess and from zero, against the oracle's
+    Thomas solve per column."""
+    from oracle import vcycle as VC
+    dt = np.float64 if precision == "fp64" else np.float32
+    s, g, levs, fac = _solver((256, 128, 40), 1.5, tbc, "LJ", precision, nlevels=(6, 5, 3))
+    lev = levs[0]
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal((40, 128, 256)).astype(dt)
+    f = rng.standard_normal((40, 128, 256)).astype(dt)
+    for which, parity in ((0, 0), (4, 1), (6, 0)):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        s.run_kernel(which, 0, 1)
+        got = s.level_field(0, 0)
+        ref = np.zeros_like(u) if which == 6 else u.copy()
+        VC.zline_half_sweep(ref, f, lev, parity, dt)
+        if which == 6:
+            m = VC.colour_mask(u.shape, parity, True)
+            got, ref = got[m], ref[m]
+        err = relerr(got, ref)
+        print(f"{precision} tbc {tbc} kernel {which} parity {parity}: err {err:.2e}", flush=True)
+        assert err < _tol(precision, "LJ"), (which, parity, err)
+    s.close()
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_zline_lean_form_full_columns(cuda, precision):
     """The benchmarked z-line sweep: the finest level of the 256^3 stretched
