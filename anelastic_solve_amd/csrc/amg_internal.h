@@ -124,10 +124,20 @@ struct PtrList {
   double* p[kMaxItems];
 };
 
+// Cells the V-cycle's prolongation may leave unchanged (DevXfer::skip):
+// those that the fine level's first post-smoothing half-sweep (colour 0)
+// rewrites from the other colour and b alone.  The z-line sweep updates the
+// cells with (i+j) even, but with periodic z it also reads its own old
+// values at rows 0 and nz-1 (the lagged wrap); the point sweep updates the
+// cells with (i+j+k) even.
+enum { kXferSkipNone = 0, kXferSkipLine = 1, kXferSkipLineWrap = 2, kXferSkipPoint = 3 };
+
 // Fine->coarse pair for transfers.
 struct DevXfer {
   int fine, coarse;
   int fx, fy, fz;
+  int skip;          // kXferSkip*: set for a 2:2:2 transfer whose fine level has a
+                     // post-smoothing sweep
 };
 
 constexpr int kBlock = 256;
@@ -248,7 +258,7 @@ inline int restrict222_blocks(const DevLevel& C) {
 void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                         hipStream_t s);
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                    hipStream_t s);
+                    hipStream_t s, bool use_skip = false);
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
 // Separable direct solve of the items' levels (u = A^-1 f), for levels with
 // nx, ny multiples of 16 and nz <= 64.  Every pass splits its output into

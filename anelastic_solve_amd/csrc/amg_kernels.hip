@@ -983,6 +983,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
 // the ghost -- the neighbour's value for Neumann, zero for Dirichlet).  A
 // factor-1 direction maps cell to cell with weight 1.  Each fine value is
 // summed nearest-first: x, then y, then z.
+//
+// skip (DevXfer::skip, kXferSkip*): the cells that the fine level's first
+// post-smoothing half-sweep rewrites from the other colour and b alone are
+// neither read nor written -- their correction would be overwritten unread,
+// so the result of that half-sweep, and of everything after it, is the same.
 struct PDir {
   int fi0, fi1;     // fine indices (fine cell 0 is nearest to coarse centre 0)
   bool fv0, fv1;    // fine cell exists
@@ -1028,9 +1033,22 @@ __device__ __forceinline__ PDir pdir(int d, int nc, int per, int bc0, int bc1) {
   return r;
 }
 
-template <class T, int FX, int FY, int FZ, class UP, class CUP>
+// Whether fine cell (h, q, a) of a 2:2:2 dual is corrected under skip rule
+// SK.  Fine cell 0 of a factor-2 direction has an odd index (2I+1) and cell
+// 1 an even one (2I+2, or 0 wrapped), so the colour of each cell is fixed by
+// its position in the dual: the skipped cells are known at compile time,
+// and their loads and stores are not in the code.
+template <int SK>
+__device__ __forceinline__ constexpr bool prolong_keeps(int h, int q, int a) {
+  return SK == kXferSkipNone ? true
+       : (SK == kXferSkipLine || SK == kXferSkipLineWrap) ? (((1 - h) + (1 - q)) & 1) != 0
+       : (((1 - h) + (1 - q) + (1 - a)) & 1) != 0;
+}
+
+template <class T, int FX, int FY, int FZ, int SK = kXferSkipNone, class UP, class CUP>
 __device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevLevel& C, CUP uc,
                                              int e) {
+  static_assert(SK == kXferSkipNone || (FX == 2 && FY == 2 && FZ == 2), "skip: 2:2:2 duals only");
   constexpr int NX = FX == 2 ? 2 : 1, NY = FY == 2 ? 2 : 1, NZ = FZ == 2 ? 2 : 1;
   const int ndx = prolong_duals_dir(C.nx, FX, F.px), ndy = prolong_duals_dir(C.ny, FY, F.py);
   const int dx = e % ndx, dyz = e / ndx, dy = dyz % ndy, dz = dyz / ndy;
@@ -1039,9 +1057,14 @@ __device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevL
   const PDir Z = pdir<FZ>(dz, C.nz, F.pz, F.bcz0, F.bcz1);
   // every load first: the coarse values v[z][y][x] (index 0: the near
   // centre of fine cell 0) and the fine cells' old values (a missing cell's
-  // index is clamped to its partner; it is not stored)
+  // index is clamped to its partner; it is not stored; a skipped cell is
+  // neither loaded nor stored)
   T v[NZ][NY][NX], fold[NZ][NY][NX];
   int fid[NZ][NY][NX];
+  bool keep[NZ][NY][NX];
+  // with periodic z the z-line sweep's lagged wrap reads its own colour at
+  // rows 0 and nz-1: both are cells of the last z dual
+  const bool zwrap = SK == kXferSkipLineWrap && Z.fi(NZ - 1) == 0;
 #pragma unroll
   for (int a = 0; a < NZ; ++a)
 #pragma unroll
@@ -1058,7 +1081,8 @@ __device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevL
         const int j = Y.fv(q) ? Y.fi(q) : Y.fi(q ^ (NY - 1));
         const int k = Z.fv(a) ? Z.fi(a) : Z.fi(a ^ (NZ - 1));
         fid[a][q][h] = cidx(i >> 1, j, k, (i + j) & 1, F.hx, F.ny);
-        fold[a][q][h] = u[fid[a][q][h]];
+        keep[a][q][h] = X.fv(h) && Y.fv(q) && Z.fv(a) && (prolong_keeps<SK>(h, q, a) || zwrap);
+        fold[a][q][h] = (prolong_keeps<SK>(h, q, a) || zwrap) ? u[fid[a][q][h]] : T(0);
       }
   // keep the old values' loads in the batch above (an empty register
   // constraint: otherwise a load is sunk into its predicated store)
@@ -1091,16 +1115,22 @@ __device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevL
 #pragma unroll
       for (int h = 0; h < NX; ++h) {
         const T add = (NZ == 2) ? T(0.75) * iy[a][q][h] + T(0.25) * iy[a ^ (NZ - 1)][q][h] : iy[a][q][h];
-        if (X.fv(h) && Y.fv(q) && Z.fv(a)) u[fid[a][q][h]] = fold[a][q][h] + add;
+        if (keep[a][q][h]) u[fid[a][q][h]] = fold[a][q][h] + add;
       }
 }
 
 template <class T, class UP, class CUP>
 __device__ __forceinline__ void prolong_any(const DevLevel& F, UP u, const DevLevel& C, CUP uc,
-                                            const DevXfer& x, int e) {
+                                            const DevXfer& x, int e, int skip = kXferSkipNone) {
   const int code = (x.fx - 1) * 4 + (x.fy - 1) * 2 + (x.fz - 1);
   switch (code) {
-    case 7: prolong_dual<T, 2, 2, 2>(F, u, C, uc, e); break;
+    case 7:
+      // the skip rules apply to 2:2:2 transfers (DevXfer::skip)
+      if (skip == kXferSkipLine) prolong_dual<T, 2, 2, 2, kXferSkipLine>(F, u, C, uc, e);
+      else if (skip == kXferSkipLineWrap) prolong_dual<T, 2, 2, 2, kXferSkipLineWrap>(F, u, C, uc, e);
+      else if (skip == kXferSkipPoint) prolong_dual<T, 2, 2, 2, kXferSkipPoint>(F, u, C, uc, e);
+      else prolong_dual<T, 2, 2, 2>(F, u, C, uc, e);
+      break;
     case 6: prolong_dual<T, 2, 2, 1>(F, u, C, uc, e); break;
     case 5: prolong_dual<T, 2, 1, 2>(F, u, C, uc, e); break;
     case 4: prolong_dual<T, 2, 1, 1>(F, u, C, uc, e); break;
@@ -1111,9 +1141,13 @@ __device__ __forceinline__ void prolong_any(const DevLevel& F, UP u, const DevLe
   }
 }
 
+// use_skip: leave the cells of the transfer's skip rule (the V-cycle's
+// prolongation, followed by the fine level's post-smoothing); 0 corrects
+// every cell.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__ Lt,
-                                                    const DevXfer* __restrict__ X, LaunchList ll) {
+                                                    const DevXfer* __restrict__ X, LaunchList ll,
+                                                    int use_skip) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevXfer x = X[ll.items[t]];
@@ -1122,7 +1156,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const DevLevel* __restrict__
   if (level_off(F)) return;
   const int e = (b - ll.prefix[t]) * kBlock + threadIdx.x;
   if (e < prolong_duals(F, C, x))
-    prolong_any<T>(F, gfield<T>(F.u), C, gfield<const T>((const void*)C.u), x, e);
+    prolong_any<T>(F, gfield<T>(F.u), C, gfield<const T>((const void*)C.u), x, e,
+                   use_skip ? x.skip : kXferSkipNone);
 }
 
 // ---------------------------------------------------------------------------
@@ -2363,9 +2398,9 @@ void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const La
   else LAUNCH(k_restrict222<float>, ll.nblocks, kBlock, L, X, ll);
 }
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                    hipStream_t s) {
-  if (fp64) LAUNCH(k_prolong<double>, ll.nblocks, kBlock, L, X, ll);
-  else LAUNCH(k_prolong<float>, ll.nblocks, kBlock, L, X, ll);
+                    hipStream_t s, bool use_skip) {
+  if (fp64) LAUNCH(k_prolong<double>, ll.nblocks, kBlock, L, X, ll, use_skip ? 1 : 0);
+  else LAUNCH(k_prolong<float>, ll.nblocks, kBlock, L, X, ll, use_skip ? 1 : 0);
 }
 size_t direct_lds_limit() {
   static size_t limit = 0;

@@ -86,10 +86,16 @@ size_t cycle_bytes(const Instance& I) {
   double b = 0.0;
   const int L = (int)I.H.lv.size();
   // levels with smoothing and transfers: above the direct level, or every
-  // level but the coarsest; a direct level reads f and writes u
+  // level but the coarsest; a direct level reads f and writes u.  The
+  // restriction reads u and b (2w per fine cell, w/8 written); the
+  // prolongation reads and writes u (2w, w/8 read), or half of it when the
+  // level's post-smoothing rewrites the other half (DevXfer::skip, 2:2:2).
   const int top = I.ld >= 0 ? I.ld : L - 1;
-  for (int l = 0; l < top; ++l)
-    b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + 4.25);
+  for (int l = 0; l < top; ++l) {
+    const HostLevel& h = I.H.lv[l];
+    const double xfer = (I.post[l] >= 1 && h.fx == 2 && h.fy == 2 && h.fz == 2) ? 3.25 : 4.25;
+    b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + xfer);
+  }
   if (I.ld >= 0) b += 2.0 * w * (double)I.H.lv[I.ld].ncell();
   const double n0 = (double)I.H.lv[0].ncell();
   b += I.fp64 ? 16.0 * n0 : 32.0 * n0;
@@ -383,6 +389,11 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     x.fine = I->lev[l];
     x.coarse = I->lev[l + 1];
     x.fx = I->H.lv[l].fx; x.fy = I->H.lv[l].fy; x.fz = I->H.lv[l].fz;
+    // the V-cycle's post-smoothing of level l starts with colour 0
+    x.skip = kXferSkipNone;
+    if (I->post[l] >= 1 && x.fx == 2 && x.fy == 2 && x.fz == 2)
+      x.skip = cfg.smoother != AMG_SMOOTHER_LJ ? kXferSkipPoint
+             : ents[l].pz ? kXferSkipLineWrap : kXferSkipLine;
     I->xf.push_back(xbase + (int)newX.size());
     newX.push_back(x);
   }
@@ -763,8 +774,10 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
         items.push_back({I->xf[dpt], cdiv(prolong_duals(hL[I->lev[dpt]], hL[I->lev[dpt + 1]],
                                                         hX[I->xf[dpt]]),
                                           kBlock)});
+    // the cells the post-smoothing's first half-sweep rewrites are left
+    // (DevXfer::skip)
     for_lists(items, [&](const LaunchList& ll, int, int) {
-      launch_prolong(fp64, L, dX, ll, stream);
+      launch_prolong(fp64, L, dX, ll, stream, true);
     });
     smooth(dpt, false, false);
   }
@@ -1197,7 +1210,7 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     items.push_back({I->lev[level], zline ? cdiv((long long)d.hx * d.ny, 64)
                                           : cdiv((long long)d.hx * d.ny * d.nz, kBlock)});
     b = 1.5 * w * d.ncell;
-  } else if (which == 1 || which == 2) {
+  } else if (which == 1 || which == 2 || which == 15) {
     if (level >= L - 1) {
       set_error("amg_run_kernel: no coarser level");
       return AMG_EINVAL;
@@ -1207,7 +1220,7 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     const int nc = which == 1 ? c.ncell : prolong_duals(d, c, hX[I->xf[level]]);
     items.push_back({I->xf[level], r222 ? restrict222_blocks(c) : cdiv(nc, kBlock)});
     if (r222) which = 9;   // the z-march restriction
-    b = (2.0 + 0.125) * w * d.ncell;
+    b = ((which == 15 && hX[I->xf[level]].skip != kXferSkipNone ? 1.0 : 2.0) + 0.125) * w * d.ncell;
   } else if (which == 3) {
     const DevLevel& fd = hL[I->fine[I->cur]];
     items.push_back({I->fine[I->cur], (int)march_blocks(fd.hx, fd.ny, fd.nz)});
@@ -1254,8 +1267,8 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
         launch_restrict(fp64, dL, dX, ll, stream);
       } else if (which == 9) {
         launch_restrict222(fp64, dL, dX, ll, stream);
-      } else if (which == 2) {
-        launch_prolong(fp64, dL, dX, ll, stream);
+      } else if (which == 2 || which == 15) {
+        launch_prolong(fp64, dL, dX, ll, stream, which == 15);
       } else if (which == 5) {
         launch_coarse(fp64, dL, ll, stream);
       } else if (fp64) {

@@ -293,6 +293,9 @@ int amg_server_drain(amg_server s);
  *         10/11: red/black z-line half-sweep in the short form (FP32)
  *         12/13: red/black z-line half-sweep in the lean form
  *         14: separable direct solve u = A^-1 f ('level' must be amg_direct_level)
+ *         15: the V-cycle's prolongation-add level+1 -> level, which leaves the
+ *             cells that the level's first post-smoothing half-sweep (red)
+ *             rewrites without reading them
  * Returns the kernel's algorithmic bytes per launch.  The caller times the
  * stream (amg_stream) with its own events. */
 int amg_run_kernel(amg_handle h, int which, int level, int precision, int reps, double* bytes);

@@ -85,6 +85,56 @@ def test_half_sweeps(cuda, case, smoother, precision):
                 assert relerr(got, ref) < _tol(precision, smoother), (lvl, which)
 
 
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("smoother", ["LJ", "RBPGS"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_prolong_skip_then_post_sweep(cuda, case, smoother, precision):
+    """The V-cycle's prolongation leaves the cells that the level's first
+    post-smoothing half-sweep (colour 0) rewrites without reading them: the
+    z-line sweep's (i+j) even cells, except rows 0 and nz-1 with periodic z
+    (its lagged wrap reads them); the point sweep's (i+j+k) even cells.
+    Followed by a red and a black half-sweep, it gives the bits of the full
+    prolongation followed by the same sweeps; on its own it corrects the
+    other cells as the full one does and leaves the skipped ones.  Only
+    2:2:2 transfers skip; the others correct every cell."""
+    n, beta, tbc, nlev = CASES[case]
+    s, g, levs, fac = _solver(n, beta, tbc, smoother, precision, rho=(case in (0, 4)), nlevels=nlev)
+    dt = np.float64 if precision == "fp64" else np.float32
+    rng = np.random.default_rng(300 + case)
+    pz = tbc[1] == 0 and tbc[3] == 0
+    for lvl in range(len(levs) - 1):
+        nx, ny, nz = levs[lvl]["n"]
+        cx, cy, cz = levs[lvl + 1]["n"]
+        u = rng.standard_normal((nz, ny, nx)).astype(dt)
+        f = rng.standard_normal((nz, ny, nx)).astype(dt)
+        uc = rng.standard_normal((cz, cy, cx)).astype(dt)
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        if smoother == "LJ":
+            skipped = ((i + j) & 1) == 0
+            if pz:
+                skipped &= (k != 0) & (k != nz - 1)
+        else:
+            skipped = ((i + j + k) & 1) == 0
+        if (nx, ny, nz) != (2 * cx, 2 * cy, 2 * cz):
+            skipped = np.zeros_like(skipped)
+        out = {}
+        for which in (2, 15):
+            s.level_field(lvl, 0, u)
+            s.level_field(lvl, 1, f)
+            s.level_field(lvl + 1, 0, uc)
+            s.run_kernel(which, lvl, 1)
+            alone = s.level_field(lvl, 0)
+            s.run_kernel(0, lvl, 1)
+            s.run_kernel(4, lvl, 1)
+            out[which] = (alone, s.level_field(lvl, 0))
+        full, skip = out[2][0], out[15][0]
+        assert np.array_equal(skip[~skipped], full[~skipped]), lvl
+        assert np.array_equal(skip[skipped], u[skipped].astype(np.float64)), lvl
+        assert skipped.any() != np.array_equal(skip, full), lvl
+        assert np.array_equal(out[15][1], out[2][1]), lvl
+    s.close()
+
+
 @pytest.mark.parametrize("n,beta,tbc,nlevels", [
     # slabs of 8 planes: the production grid's finest level (Dirichlet z walls)
     ((256, 64, 256), 0.0, (0, 2, 0, 2, 0, 0), None),
