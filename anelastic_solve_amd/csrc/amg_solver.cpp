@@ -89,11 +89,13 @@ size_t cycle_bytes(const Instance& I) {
   // level but the coarsest; a direct level reads f and writes u.  The
   // restriction reads u and b (2w per fine cell, w/8 written); the
   // prolongation reads and writes u (2w, w/8 read), or half of it when the
-  // level's post-smoothing rewrites the other half (DevXfer::skip, 2:2:2).
+  // level's post-smoothing rewrites the other half (DevXfer::skip, 2:2:2;
+  // the V-cycle's own transfers, above the coarse tail).
   const int top = I.ld >= 0 ? I.ld : L - 1;
   for (int l = 0; l < top; ++l) {
     const HostLevel& h = I.H.lv[l];
-    const double xfer = (I.post[l] >= 1 && h.fx == 2 && h.fy == 2 && h.fz == 2) ? 3.25 : 4.25;
+    const bool half = l < I.lc() && I.post[l] >= 1 && h.fx == 2 && h.fy == 2 && h.fz == 2;
+    const double xfer = half ? 3.25 : 4.25;
     b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + xfer);
   }
   if (I.ld >= 0) b += 2.0 * w * (double)I.H.lv[I.ld].ncell();
