@@ -130,6 +130,11 @@ struct PtrList {
 // cells with (i+j) even, but with periodic z it also reads its own old
 // values at rows 0 and nz-1 (the lagged wrap); the point sweep updates the
 // cells with (i+j+k) even.
+// Cells whose residual the V-cycle's restriction takes as zero
+// (DevXfer::rskip, Line or Point): those of colour 1, whose equations the
+// last half-sweep of the fine level's pre-smoothing has just solved (the
+// z-line sweep's lagged periodic-z wrap leaves rows 0 and nz-1 unsolved, so
+// a periodic-z level keeps every residual).
 enum { kXferSkipNone = 0, kXferSkipLine = 1, kXferSkipLineWrap = 2, kXferSkipPoint = 3 };
 
 // Fine->coarse pair for transfers.
@@ -138,6 +143,8 @@ struct DevXfer {
   int fx, fy, fz;
   int skip;          // kXferSkip*: set for a 2:2:2 transfer whose fine level has a
                      // post-smoothing sweep
+  int rskip;         // kXferSkip*: set for a marching restriction (restrict222_ok)
+                     // whose fine level has a pre-smoothing sweep
 };
 
 constexpr int kBlock = 256;
@@ -256,7 +263,7 @@ inline int restrict222_blocks(const DevLevel& C) {
   return (int)(((long long)C.nx * C.ny + kBlock - 1) / kBlock) * ((C.nz + kr - 1) / kr);
 }
 void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                        hipStream_t s);
+                        hipStream_t s, bool use_skip = false);
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s, bool use_skip = false);
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);

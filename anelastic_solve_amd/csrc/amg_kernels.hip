@@ -853,24 +853,22 @@ __global__ __launch_bounds__(kBlock) void k_restrict(const DevLevel* __restrict_
 // residual is formed with offsum/diag_at as in restrict_elem, and the coarse
 // value sums its eight in the same order (z, then y, then x), so the result
 // is restrict_elem's.
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restrict__ Lt,
-                                                        const DevXfer* __restrict__ X,
-                                                        LaunchList ll) {
-  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int t = find_item(ll.prefix, ll.nitems, b);
-  const DevXfer x = X[ll.items[t]];
-  const DevLevel& C = Lt[x.coarse];
-  const DevLevel& F = Lt[x.fine];
-  if (level_off(F)) return;
-  const int cols = C.nx * C.ny;
-  const int ncb = (cols + kBlock - 1) / kBlock;
-  const int lb = b - ll.prefix[t];
-  const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
-  if (col >= cols) return;
-  const int J = col / C.nx, I = col - J * C.nx;
-  const int kr = restrict222_k(C);
-  const int K0 = bk * kr, K1 = min(K0 + kr, C.nz);
+//
+// SK (DevXfer::rskip): after the fine level's pre-smoothing, whose last
+// half-sweep solved the equations of its colour-1 cells from their
+// neighbours, those cells' residuals are zero but for rounding; they are
+// taken as zero, and the loads only they need (their b and their outside
+// neighbours) are not in the code.  Colour 1 of cell (2I+cx, 2J+cy, 2K+h):
+// (cx+cy) odd for the z-line sweep, (cx+cy+h) odd for the point sweep.
+template <int SK>
+__device__ __forceinline__ constexpr bool restrict_keeps(int cx, int cy, int h) {
+  return SK == kXferSkipLine ? ((cx + cy) & 1) == 0
+       : SK == kXferSkipPoint ? ((cx + cy + h) & 1) == 0 : true;
+}
+
+template <class T, int SK>
+__device__ __forceinline__ void restrict222_cols(const DevLevel& F, const DevLevel& C, int I, int J,
+                                                 int K0, int K1) {
   const auto u = gfield<const T>((const void*)F.u);
   const auto f = gfield<const T>((const void*)F.f);
   const auto co = coefs<T>(F);
@@ -925,12 +923,14 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
       const int kk = ka + h;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        fv[h][q][0] = f[kk * plane + o[q][0]];
-        fv[h][q][1] = f[kk * plane + o[q][1]];
-        xm[h][q] = u[kk * plane + oxm[q]];
-        xp[h][q] = u[kk * plane + oxp[q]];
-        ym[h][q] = u[kk * plane + oym[q]];
-        yp[h][q] = u[kk * plane + oyp[q]];
+        // fv[h][cy][cx]; xm/xp[h][cy] serve cells (0, cy)/(1, cy), ym/yp[h][cx]
+        // cells (cx, 0)/(cx, 1)
+        fv[h][q][0] = restrict_keeps<SK>(0, q, h) ? f[kk * plane + o[q][0]] : T(0);
+        fv[h][q][1] = restrict_keeps<SK>(1, q, h) ? f[kk * plane + o[q][1]] : T(0);
+        xm[h][q] = restrict_keeps<SK>(0, q, h) ? u[kk * plane + oxm[q]] : T(0);
+        xp[h][q] = restrict_keeps<SK>(1, q, h) ? u[kk * plane + oxp[q]] : T(0);
+        ym[h][q] = restrict_keeps<SK>(q, 0, h) ? u[kk * plane + oym[q]] : T(0);
+        yp[h][q] = restrict_keeps<SK>(q, 1, h) ? u[kk * plane + oyp[q]] : T(0);
       }
     }
     T acc = T(0), vs = T(0);
@@ -946,14 +946,16 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
 #pragma unroll
         for (int cx = 0; cx < 2; ++cx) {
           const int i = 2 * I + cx;
-          const T uxm = cx ? cc[cy][0] : xm[h][cy];
-          const T uxp = cx ? xp[h][cy] : cc[cy][1];
-          const T uym = cy ? cc[0][cx] : ym[h][cx];
-          const T uyp = cy ? yp[h][cx] : cc[1][cx];
-          const T s = offsum<T>(co, F, i, j, k, uxm, uxp, uym, uyp, zm[cy][cx], zp[cy][cx]);
-          const T rr = fv[h][cy][cx] - (s + diag_at<T>(co, F, i, j, k) * cc[cy][cx]);
           const T v = T(vz[k] * vy[j] * vx[i]);
-          acc += v * rr;
+          if (restrict_keeps<SK>(cx, cy, h)) {
+            const T uxm = cx ? cc[cy][0] : xm[h][cy];
+            const T uxp = cx ? xp[h][cy] : cc[cy][1];
+            const T uym = cy ? cc[0][cx] : ym[h][cx];
+            const T uyp = cy ? yp[h][cx] : cc[1][cx];
+            const T s = offsum<T>(co, F, i, j, k, uxm, uxp, uym, uyp, zm[cy][cx], zp[cy][cx]);
+            const T rr = fv[h][cy][cx] - (s + diag_at<T>(co, F, i, j, k) * cc[cy][cx]);
+            acc += v * rr;
+          }
           vs += v;
         }
       }
@@ -967,6 +969,32 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
         uc[cy][cx] = ub[cy][cx];
       }
   }
+}
+
+// use_skip: take the transfer's rskip rule (the V-cycle's restriction after
+// pre-smoothing); 0 forms every residual.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restrict__ Lt,
+                                                        const DevXfer* __restrict__ X,
+                                                        LaunchList ll, int use_skip) {
+  const int b = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int t = find_item(ll.prefix, ll.nitems, b);
+  const DevXfer x = X[ll.items[t]];
+  const DevLevel& C = Lt[x.coarse];
+  const DevLevel& F = Lt[x.fine];
+  if (level_off(F)) return;
+  const int cols = C.nx * C.ny;
+  const int ncb = (cols + kBlock - 1) / kBlock;
+  const int lb = b - ll.prefix[t];
+  const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
+  if (col >= cols) return;
+  const int J = col / C.nx, I = col - J * C.nx;
+  const int kr = restrict222_k(C);
+  const int K0 = bk * kr, K1 = min(K0 + kr, C.nz);
+  const int sk = use_skip ? x.rskip : kXferSkipNone;
+  if (sk == kXferSkipLine) restrict222_cols<T, kXferSkipLine>(F, C, I, J, K0, K1);
+  else if (sk == kXferSkipPoint) restrict222_cols<T, kXferSkipPoint>(F, C, I, J, K0, K1);
+  else restrict222_cols<T, kXferSkipNone>(F, C, I, J, K0, K1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2393,9 +2421,9 @@ void launch_restrict(bool fp64, const DevLevel* L, const DevXfer* X, const Launc
   else LAUNCH(k_restrict<float>, ll.nblocks, kBlock, L, X, ll);
 }
 void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
-                        hipStream_t s) {
-  if (fp64) LAUNCH(k_restrict222<double>, ll.nblocks, kBlock, L, X, ll);
-  else LAUNCH(k_restrict222<float>, ll.nblocks, kBlock, L, X, ll);
+                        hipStream_t s, bool use_skip) {
+  if (fp64) LAUNCH(k_restrict222<double>, ll.nblocks, kBlock, L, X, ll, use_skip ? 1 : 0);
+  else LAUNCH(k_restrict222<float>, ll.nblocks, kBlock, L, X, ll, use_skip ? 1 : 0);
 }
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s, bool use_skip) {

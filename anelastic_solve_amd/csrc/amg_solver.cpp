@@ -81,7 +81,8 @@ size_t partial_blocks(const Instance& I) {
   return (size_t)b + 1;
 }
 
-size_t cycle_bytes(const Instance& I) {
+size_t cycle_bytes(const Instance& I, const std::vector<DevXfer>& X) {
+  // X: the instance's own transfers, X[l] from level l to level l+1
   const double w = I.fp64 ? 8.0 : 4.0;
   double b = 0.0;
   const int L = (int)I.H.lv.size();
@@ -89,13 +90,16 @@ size_t cycle_bytes(const Instance& I) {
   // level but the coarsest; a direct level reads f and writes u.  The
   // restriction reads u and b (2w per fine cell, w/8 written); the
   // prolongation reads and writes u (2w, w/8 read), or half of it when the
-  // level's post-smoothing rewrites the other half (DevXfer::skip, 2:2:2;
-  // the V-cycle's own transfers, above the coarse tail).
+  // level's post-smoothing rewrites the other half (DevXfer::skip, 2:2:2);
+  // the marching restriction skips b and the outside neighbours of the
+  // cells whose residual the pre-smoothing solved (DevXfer::rskip: w/2
+  // less).  Both apply to the V-cycle's own transfers, above the coarse tail.
   const int top = I.ld >= 0 ? I.ld : L - 1;
   for (int l = 0; l < top; ++l) {
-    const HostLevel& h = I.H.lv[l];
-    const bool half = l < I.lc() && I.post[l] >= 1 && h.fx == 2 && h.fy == 2 && h.fz == 2;
-    const double xfer = half ? 3.25 : 4.25;
+    const DevXfer& x = X[(size_t)l];
+    const bool cyc = l < I.lc();
+    const double xfer = (cyc && x.rskip != kXferSkipNone ? 1.625 : 2.125) +
+                        (cyc && x.skip != kXferSkipNone ? 1.125 : 2.125);
     b += w * (double)I.H.lv[l].ncell() * (3.0 * (I.pre[l] + I.post[l]) + xfer);
   }
   if (I.ld >= 0) b += 2.0 * w * (double)I.H.lv[I.ld].ncell();
@@ -391,11 +395,15 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     x.fine = I->lev[l];
     x.coarse = I->lev[l + 1];
     x.fx = I->H.lv[l].fx; x.fy = I->H.lv[l].fy; x.fz = I->H.lv[l].fz;
-    // the V-cycle's post-smoothing of level l starts with colour 0
+    // the V-cycle's post-smoothing of level l starts with colour 0, its
+    // pre-smoothing ends with colour 1
+    const bool line = cfg.smoother == AMG_SMOOTHER_LJ;
     x.skip = kXferSkipNone;
     if (I->post[l] >= 1 && x.fx == 2 && x.fy == 2 && x.fz == 2)
-      x.skip = cfg.smoother != AMG_SMOOTHER_LJ ? kXferSkipPoint
-             : ents[l].pz ? kXferSkipLineWrap : kXferSkipLine;
+      x.skip = !line ? kXferSkipPoint : ents[l].pz ? kXferSkipLineWrap : kXferSkipLine;
+    x.rskip = kXferSkipNone;
+    if (I->pre[l] >= 1 && restrict222_ok(ents[l], ents[l + 1], x) && !(line && ents[l].pz))
+      x.rskip = line ? kXferSkipLine : kXferSkipPoint;
     I->xf.push_back(xbase + (int)newX.size());
     newX.push_back(x);
   }
@@ -545,7 +553,7 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     if (!I->d_met) return fail();
     AMG_HIP(hipMemcpy(I->d_met, m.data(), m.size() * 8, hipMemcpyHostToDevice));
   }
-  I->bytes_per_cycle = cycle_bytes(*I);
+  I->bytes_per_cycle = cycle_bytes(*I, newX);
   I->bytes_corr = (size_t)((I->fp64 ? 16.0 : 32.0) * (double)I->H.lv[0].ncell());
   I->bytes_vcycle = I->bytes_per_cycle - I->bytes_corr;
   // paired cycles: mixed precision, the residual stop rule, and a finest
@@ -715,8 +723,10 @@ void Context::vcycle(const std::vector<Instance*>& S, bool fp64) {
       if (restrict222_ok(hL[I->lev[dpt]], C, hX[xi])) items222.push_back({xi, restrict222_blocks(C)});
       else items.push_back({xi, cdiv(C.ncell, kBlock)});
     }
+    // the residuals the pre-smoothing's last half-sweep solved are taken as
+    // zero (DevXfer::rskip)
     for_lists(items222, [&](const LaunchList& ll, int, int) {
-      launch_restrict222(fp64, L, dX, ll, stream);
+      launch_restrict222(fp64, L, dX, ll, stream, true);
     });
     for_lists(items, [&](const LaunchList& ll, int, int) {
       launch_restrict(fp64, L, dX, ll, stream);
@@ -1212,17 +1222,22 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
     items.push_back({I->lev[level], zline ? cdiv((long long)d.hx * d.ny, 64)
                                           : cdiv((long long)d.hx * d.ny * d.nz, kBlock)});
     b = 1.5 * w * d.ncell;
-  } else if (which == 1 || which == 2 || which == 15) {
+  } else if (which == 1 || which == 2 || which == 15 || which == 16) {
     if (level >= L - 1) {
       set_error("amg_run_kernel: no coarser level");
       return AMG_EINVAL;
     }
     const DevLevel& c = hL[I->lev[level + 1]];
-    const bool r222 = which == 1 && restrict222_ok(d, c, hX[I->xf[level]]);
-    const int nc = which == 1 ? c.ncell : prolong_duals(d, c, hX[I->xf[level]]);
+    const DevXfer& xf = hX[I->xf[level]];
+    const bool restr = which == 1 || which == 16;
+    const bool r222 = restr && restrict222_ok(d, c, xf);
+    const int nc = restr ? c.ncell : prolong_duals(d, c, xf);
     items.push_back({I->xf[level], r222 ? restrict222_blocks(c) : cdiv(nc, kBlock)});
-    if (r222) which = 9;   // the z-march restriction
-    b = ((which == 15 && hX[I->xf[level]].skip != kXferSkipNone ? 1.0 : 2.0) + 0.125) * w * d.ncell;
+    double fine = 2.0;
+    if (which == 15 && xf.skip != kXferSkipNone) fine = 1.0;
+    if (which == 16 && r222 && xf.rskip != kXferSkipNone) fine = 1.5;
+    b = (fine + 0.125) * w * d.ncell;
+    if (r222) which = which == 16 ? 17 : 9;   // the z-march restriction (17: with rskip)
   } else if (which == 3) {
     const DevLevel& fd = hL[I->fine[I->cur]];
     items.push_back({I->fine[I->cur], (int)march_blocks(fd.hx, fd.ny, fd.nz)});
@@ -1265,10 +1280,10 @@ int Context::run_kernel(Instance* I, int which, int level, int reps, double* byt
         else rb_launches(items, fp64, par, zg);
       } else if (which == 7 || which == 8) {
         launch_zline_col(fp64, dL, ll, which == 8 ? 1 : 0, 0, stream);
-      } else if (which == 1) {
+      } else if (which == 1 || which == 16) {
         launch_restrict(fp64, dL, dX, ll, stream);
-      } else if (which == 9) {
-        launch_restrict222(fp64, dL, dX, ll, stream);
+      } else if (which == 9 || which == 17) {
+        launch_restrict222(fp64, dL, dX, ll, stream, which == 17);
       } else if (which == 2 || which == 15) {
         launch_prolong(fp64, dL, dX, ll, stream, which == 15);
       } else if (which == 5) {

@@ -149,7 +149,9 @@ struct Context {
 // hierarchy; no device work.  AMG_OK or an error code with the message set.
 int check_grid(const amg_config& cfg, const amg_grid& g, Hierarchy& H);
 
-size_t cycle_bytes(const Instance& I);
+// Algorithmic bytes of one V-cycle and its FP64 correction; X holds the
+// instance's transfers, X[l] from level l to level l+1.
+size_t cycle_bytes(const Instance& I, const std::vector<DevXfer>& X);
 // Blocks of the instance's largest finest-level launch that writes block
 // partials (each writes at most 4 values per block).
 size_t partial_blocks(const Instance& I);

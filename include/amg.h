@@ -296,6 +296,10 @@ int amg_server_drain(amg_server s);
  *         15: the V-cycle's prolongation-add level+1 -> level, which leaves the
  *             cells that the level's first post-smoothing half-sweep (red)
  *             rewrites without reading them
+ *         16: the V-cycle's residual + restriction level -> level+1, which
+ *             takes as zero the residuals that the level's last
+ *             pre-smoothing half-sweep (black) solved (marching restriction
+ *             only; otherwise as 1)
  * Returns the kernel's algorithmic bytes per launch.  The caller times the
  * stream (amg_stream) with its own events. */
 int amg_run_kernel(amg_handle h, int which, int level, int precision, int reps, double* bytes);

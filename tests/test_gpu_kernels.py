@@ -198,6 +198,57 @@ def test_restriction_march_large_level(cuda, precision, tbc):
     s.close()
 
 
+@pytest.mark.parametrize("smoother,tbc", [
+    ("LJ", (0, 1, 0, 1, 0, 0)), ("LJ", (0, 2, 0, 2, 0, 0)), ("RBPGS", (0, 1, 0, 1, 0, 0)),
+    ("LJ", (0, 0, 0, 0, 0, 0))])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_restriction_skips_solved_residuals(cuda, precision, smoother, tbc):
+    """The V-cycle's marching restriction takes as zero the residuals of the
+    cells that the level's last pre-smoothing half-sweep (colour 1) has just
+    solved: (i+j) odd for the z-line sweep, (i+j+k) odd for the point sweep;
+    with periodic z (the z-line sweep's lagged wrap) it forms every residual.
+    On random fields it is the oracle's restriction of the residual with
+    those cells zeroed; after a red and a black half-sweep it is the full
+    restriction to rounding."""
+    from oracle import vcycle as VC
+    nz = 72
+    s, g, levs, fac = _solver((256, 256, nz), 1.5, tbc, smoother, precision, rho=True,
+                              nlevels=(7, 7, 2))
+    assert tuple(fac[0]) == (2, 2, 2) and tuple(levs[1]["n"]) == (128, 128, nz // 2)
+    dt = np.float64 if precision == "fp64" else np.float32
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    pz = tbc[1] == 0 and tbc[3] == 0
+    rng = np.random.default_rng(310)
+    u = rng.standard_normal((nz, 256, 256)).astype(dt)
+    f = rng.standard_normal((nz, 256, 256)).astype(dt)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(256), np.arange(256), indexing="ij")
+    solved = (((i + j) if smoother == "LJ" else (i + j + k)) & 1) == 1
+    if smoother == "LJ" and pz:
+        solved[:] = False
+
+    def restriction(which, sweep):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        if sweep:
+            s.run_kernel(0, 0, 1)
+            s.run_kernel(4, 0, 1)
+        s.run_kernel(which, 0, 1)
+        return s.level_field(1, 1)
+
+    r = VC.residual(u, f, levs[0], dt)
+    r[solved] = 0
+    ref = VC.restrict(r, levs[0], levs[1], fac[0], dt)
+    got = restriction(16, False)
+    assert relerr(got, ref) < tol
+    if not solved.any():
+        assert np.array_equal(got, restriction(1, False))
+    full, skip = restriction(1, True), restriction(16, True)
+    err = relerr(skip, full)
+    print(f"{precision} {smoother} tbc {tbc}: after the sweeps {err:.2e}", flush=True)
+    assert err < (1e-12 if precision == "fp64" else 1e-5), err
+    s.close()
+
+
 @pytest.mark.parametrize("case", range(len(CASES)))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_transfers_and_coarse_solve(cuda, case, precision):
