@@ -58,7 +58,9 @@ struct DevLevel {
   void* f;                   // right-hand side (chequer layout)
   void* scr;                 // z-line scratch: nz * hx * ny values
   // separable direct solve of a coarse level (amg_direct.cpp, k_direct_*)
-  const double* dmat;        // [Vx^-T | Vx^T | Vy^-1 | Vy | Vz^-1 | Vz], FP64 (z padded to 16)
+  // the direct level's tables, in its compute type (FP64 for an FP64
+  // V-cycle, FP32 -- stored behind these pointers -- for a mixed solve's):
+  const double* dmat;        // [Vx^-T | Vx^T | Vy^-1 | Vy | Vz^-1 | Vz] (z padded to 16)
   const double* dinv;        // 1 / (lx + ly + lz) per eigenmode, [k][g][e]
   double* dscr;              // transformed right-hand side, then solution: 2 * nz*ny*nx (mode fastest)
 };
@@ -267,16 +269,19 @@ void launch_restrict222(bool fp64, const DevLevel* L, const DevXfer* X, const La
 void launch_prolong(bool fp64, const DevLevel* L, const DevXfer* X, const LaunchList& ll,
                     hipStream_t s, bool use_skip = false);
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s);
-// Separable direct solve of the items' levels (u = A^-1 f), for levels with
-// nx, ny multiples of 16 and nz <= 64.  Every pass splits its output into
-// slabs of kDirectCols columns, one workgroup each (a level of 64^3 fills
-// the device with 256): 'planes' has nz*nx/kDirectCols blocks per item,
-// 'slabs' nx*ny/kDirectCols; lds = the largest direct_lds() of the items,
-// at most direct_lds_limit().
+// Separable direct solve of the items' levels (u = A^-1 f), for levels of
+// at most kDirectCells cells with nx, ny multiples of 16 and nz <= 64,
+// computed in FP64 for an FP64 V-cycle and in FP32 for the FP32 V-cycle of
+// a mixed solve.  Every pass splits its output into slabs of kDirectCols
+// columns, one workgroup each (a level of 64^3 fills the device with 256):
+// 'planes' has nz*nx/kDirectCols blocks per item, 'slabs' nx*ny/kDirectCols;
+// lds = the largest direct_lds() of the items, at most direct_lds_limit().
+// (A larger FP32 level fits the LDS -- 128^3 does -- but its passes take
+// longer than the levels they would replace.)
 constexpr long long kDirectCells = 1 << 18;
 constexpr int kDirectCols = 16;
 constexpr int kDirectMaxNz = 64;
-size_t direct_lds(int nx, int ny, int nz);
+size_t direct_lds(int nx, int ny, int nz, bool fp64);
 size_t direct_lds_limit();
 void launch_direct(bool fp64, const DevLevel* L, const LaunchList& planes, const LaunchList& slabs,
                    size_t lds, hipStream_t s);

@@ -2155,53 +2155,73 @@ __global__ __launch_bounds__(kDivFin) void k_divergence_finalize(const double* _
 // Separable direct solve of a coarse level (host set-up in amg_direct.cpp):
 // u = A^-1 f with A = Ax + Ay + Az and every 1-D operator diagonalised,
 // A_d = V_d diag(l_d) V_d^-1.  Three passes, all dense products on the
-// matrix cores (v_mfma_f64_16x16x4f64), FP64 whatever the level's
-// precision:
-//   k_direct_fwd  per plane k:      f^ = Vy^-1 f_k Vx^-T          (-> dscr)
-//   k_direct_z    per 64 modes:     x^ = Vz (inv .* (Vz^-1 f^))    (-> dscr + n)
-//   k_direct_inv  per plane k:      u_k = Vy x^_k Vx^T
+// matrix cores, in the compute type CT of the level's tables: FP64
+// (v_mfma_f64_16x16x4f64) for an FP64 V-cycle, FP32 (v_mfma_f32_16x16x4f32)
+// for the FP32 V-cycle of a mixed solve, whose coarse correction needs no
+// more (at 64^3 the FP32 passes are the faster):
+//   k_direct_xy<FWD>  per plane k:      f^ = Vy^-1 f_k Vx^-T          (-> dscr)
+//   k_direct_z        per 16 modes:     x^ = Vz (inv .* (Vz^-1 f^))    (-> dscr + n)
+//   k_direct_xy<inv>  per plane k:      u_k = Vy x^_k Vx^T
 // with inv = 1/(lx + ly + lz) per eigenmode (0 for a singular operator's
 // null mode: the solution then has sum V u = 0).  Operands are staged in
 // LDS; rows are padded by one word against bank conflicts.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
-// LDS of the plane passes and of the z pass (doubles -> bytes)
-__host__ __device__ inline size_t direct_lds_xy(int nx, int ny) {
-  return ((size_t)ny * (nx + 1) + (size_t)ny * (ny + 1) + (size_t)(nx + ny) * (kDirectCols + 1)) *
-         sizeof(double);
+// The 16x16x4 product of each compute type: lane l holds A[i][k] and B[k][j]
+// with i, j = l % 16 and k = l / 16; result register r of lane l holds
+// D[row(l / 16, r)][l % 16] (the two types lay D out differently).
+template <class CT> struct Mfma16;
+template <> struct Mfma16<double> {
+  typedef dbl4 V;
+  static __device__ __forceinline__ V op(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct Mfma16<float> {
+  typedef flt4 V;
+  static __device__ __forceinline__ V op(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
+
+// LDS of the plane passes and of the z pass (es: bytes per element)
+__host__ __device__ inline size_t direct_lds_xy(int nx, int ny, size_t es) {
+  return ((size_t)ny * (nx + 1) + (size_t)ny * (ny + 1) + (size_t)(nx + ny) * (kDirectCols + 1)) * es;
 }
-__host__ __device__ inline size_t direct_lds_z(int nzp) {
-  return ((size_t)2 * nzp * (kDirectCols + 1) + (size_t)2 * nzp * (nzp + 1)) * sizeof(double);
+__host__ __device__ inline size_t direct_lds_z(int nzp, size_t es) {
+  return ((size_t)2 * nzp * (kDirectCols + 1) + (size_t)2 * nzp * (nzp + 1)) * es;
 }
 __host__ __device__ inline int direct_nzp(int nz) { return (nz + 15) & ~15; }
 
 // C = A B for an nr x nc output, A (nr x nq, row stride as) and B (nq x nc,
 // row stride bs) in LDS; nr, nc, nq multiples of 16.  A wave takes one
-// 16x16 tile at a time: per v_mfma_f64_16x16x4f64, lane l holds A[i][k] and
-// B[k][j] with i, j = l % 16 and k = l / 16, and after the products D[i][j]
-// for j = l % 16, i = l / 16 + 4 r in register r.  The operands of four
-// products are read before them.
-template <class AP, class BP, class F>
+// 16x16 tile at a time (Mfma16 lane layout).  The operands of four products
+// are read before them.
+template <class CT, class AP, class BP, class F>
 __device__ __forceinline__ void mfma_products(AP A, int as, BP B, int bs, int nr, int nc, int nq,
                                               F&& store) {
+  typedef Mfma16<CT> M;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int tr = nr >> 4, tc = nc >> 4;
   for (int tile = wave; tile < tr * tc; tile += nw) {
     const int r0 = (tile / tc) << 4, c0 = (tile % tc) << 4;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    typename M::V acc = {CT(0), CT(0), CT(0), CT(0)};
     for (int q0 = 0; q0 < nq; q0 += 16) {
-      double av[4], bv[4];
+      CT av[4], bv[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         av[q] = A[(r0 + li) * as + q0 + 4 * q + lk];
         bv[q] = B[(q0 + 4 * q + lk) * bs + c0 + li];
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc, 0, 0, 0);
+      for (int q = 0; q < 4; ++q) acc = M::op(av[q], bv[q], acc);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) store(r0 + lk + 4 * r, c0 + li, acc[r]);
+    for (int r = 0; r < 4; ++r) store(r0 + M::row(lk, r), c0 + li, acc[r]);
   }
 }
 
@@ -2209,13 +2229,13 @@ __device__ __forceinline__ void mfma_products(AP A, int as, BP B, int bs, int nr
 // per thread before any store (a loop of load-store pairs would wait out a
 // memory round trip per element): store_a(e, load_a(e)) for e < na, and
 // the same for b and c.
-template <int N, class LA, class SA, class LB, class SB, class LC, class SC>
+template <int N, class CT, class LA, class SA, class LB, class SB, class LC, class SC>
 __device__ __forceinline__ void stage3(int na, LA&& la, SA&& sa, int nb, LB&& lb, SB&& sb, int nc,
                                        LC&& lc, SC&& sc) {
   const int bd = blockDim.x, step = N * bd;
   const int n = max(na, max(nb, nc));
   for (int e0 = threadIdx.x; e0 < n; e0 += step) {
-    double va[N], vb[N], vc[N];
+    CT va[N], vb[N], vc[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
       const int e = e0 + q * bd;
@@ -2235,9 +2255,10 @@ __device__ __forceinline__ void stage3(int na, LA&& la, SA&& sa, int nb, LB&& lb
 
 // The plane passes, one workgroup per (plane k, slab of kDirectCols columns
 // c0..): FWD: the slab of f^_k = Vy^-1 f_k Vx^-T into dscr, from plane k of
-// f (level precision); otherwise the slab of u_k = Vy x^_k Vx^T from dscr + n.
-// Either is My (X Mx[:, slab]) with the whole plane X staged.
-template <class T, bool FWD>
+// f (level precision T); otherwise the slab of u_k = Vy x^_k Vx^T from
+// dscr + n.  Either is My (X Mx[:, slab]) with the whole plane X staged.
+// The tables (dmat, dinv, dscr) hold CT values.
+template <class T, class CT, bool FWD>
 __global__ __launch_bounds__(kBlock) void k_direct_xy(const DevLevel* __restrict__ Lt, LaunchList ll) {
   extern __shared__ double dl[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -2247,38 +2268,40 @@ __global__ __launch_bounds__(kBlock) void k_direct_xy(const DevLevel* __restrict
   const int nx = L.nx, ny = L.ny, hx = L.hx, sx = nx + 1, sy = ny + 1;
   constexpr int sc = kDirectCols + 1;
   const int ns = nx / kDirectCols, q = b - ll.prefix[t], k = q / ns, c0 = (q - k * ns) * kDirectCols;
-  const sp<kS, double> X = (sp<kS, double>)dl;   // the plane, ny x nx (stride sx)
-  const sp<kS, double> My = X + ny * sx;         // FWD: Vy^-1, else Vy (ny x ny, stride sy)
-  const sp<kS, double> Mx = My + ny * sy;        // columns c0.. of Vx^-T / Vx^T (nx x 16)
-  const sp<kS, double> Y = Mx + nx * sc;         // X Mx (ny x 16)
-  const auto m = gp(L.dmat);
+  const sp<kS, CT> X = (sp<kS, CT>)(sp<kS, double>)dl;   // the plane, ny x nx (stride sx)
+  const sp<kS, CT> My = X + ny * sx;         // FWD: Vy^-1, else Vy (ny x ny, stride sy)
+  const sp<kS, CT> Mx = My + ny * sy;        // columns c0.. of Vx^-T / Vx^T (nx x 16)
+  const sp<kS, CT> Y = Mx + nx * sc;         // X Mx (ny x 16)
+  const auto m = gp(reinterpret_cast<const CT*>(L.dmat));
   const size_t ox = FWD ? 0 : (size_t)nx * nx;
   const size_t oy = (size_t)2 * nx * nx + (FWD ? 0 : (size_t)ny * ny);
   auto lmx = [&](int e) { return m[ox + (size_t)(e / kDirectCols) * nx + c0 + e % kDirectCols]; };
-  auto smx = [&](int e, double v) { Mx[(e / kDirectCols) * sc + e % kDirectCols] = v; };
+  auto smx = [&](int e, CT v) { Mx[(e / kDirectCols) * sc + e % kDirectCols] = v; };
   auto lmy = [&](int e) { return m[oy + e]; };
-  auto smy = [&](int e, double v) { My[(e / ny) * sy + e % ny] = v; };
-  auto sxp = [&](int e, double v) { X[(e / nx) * sx + e % nx] = v; };
+  auto smy = [&](int e, CT v) { My[(e / ny) * sy + e % ny] = v; };
+  auto sxp = [&](int e, CT v) { X[(e / nx) * sx + e % nx] = v; };
+  const auto scr = gp(reinterpret_cast<CT*>(L.dscr));
   if constexpr (FWD) {
     const auto f = gfield<const T>((const void*)L.f);
     auto lf = [&](int e) {
       const int j = e / nx, i = e - j * nx;
-      return (double)f[cidx(i >> 1, j, k, (i + j) & 1, hx, ny)];
+      return (CT)f[cidx(i >> 1, j, k, (i + j) & 1, hx, ny)];
     };
-    stage3<16>(nx * ny, lf, sxp, ny * ny, lmy, smy, nx * kDirectCols, lmx, smx);
+    stage3<16, CT>(nx * ny, lf, sxp, ny * ny, lmy, smy, nx * kDirectCols, lmx, smx);
   } else {
-    const auto in = gp((const double*)L.dscr) + (size_t)L.ncell + (size_t)k * ny * nx;
-    stage3<16>(nx * ny, [&](int e) { return in[e]; }, sxp, ny * ny, lmy, smy, nx * kDirectCols, lmx, smx);
+    const auto in = scr + (size_t)L.ncell + (size_t)k * ny * nx;
+    stage3<16, CT>(nx * ny, [&](int e) { return in[e]; }, sxp, ny * ny, lmy, smy, nx * kDirectCols, lmx,
+                   smx);
   }
   __syncthreads();
-  mfma_products(X, sx, Mx, sc, ny, kDirectCols, nx, [&](int r, int c, double v) { Y[r * sc + c] = v; });
+  mfma_products<CT>(X, sx, Mx, sc, ny, kDirectCols, nx, [&](int r, int c, CT v) { Y[r * sc + c] = v; });
   __syncthreads();
   if constexpr (FWD) {
-    const auto out = gp(L.dscr) + (size_t)k * ny * nx + c0;
-    mfma_products(My, sy, Y, sc, ny, kDirectCols, ny, [&](int r, int c, double v) { out[r * nx + c] = v; });
+    const auto out = scr + (size_t)k * ny * nx + c0;
+    mfma_products<CT>(My, sy, Y, sc, ny, kDirectCols, ny, [&](int r, int c, CT v) { out[r * nx + c] = v; });
   } else {
     const auto u = gfield<T>(L.u);
-    mfma_products(My, sy, Y, sc, ny, kDirectCols, ny, [&](int r, int c, double v) {
+    mfma_products<CT>(My, sy, Y, sc, ny, kDirectCols, ny, [&](int r, int c, CT v) {
       const int i = c0 + c;
       u[cidx(i >> 1, r, k, (i + r) & 1, hx, ny)] = (T)v;
     });
@@ -2287,6 +2310,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_xy(const DevLevel* __restrict
 
 // The z pass, one workgroup per slab of kDirectCols modes m0.. (all k):
 // x^ = Vz (inv .* (Vz^-1 f^)).
+template <class CT>
 __global__ __launch_bounds__(kBlock) void k_direct_z(const DevLevel* __restrict__ Lt, LaunchList ll) {
   extern __shared__ double dl[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -2296,30 +2320,30 @@ __global__ __launch_bounds__(kBlock) void k_direct_z(const DevLevel* __restrict_
   const int nx = L.nx, ny = L.ny, nz = L.nz, nzp = direct_nzp(nz);
   const int plane = nx * ny, m0 = (b - ll.prefix[t]) * kDirectCols, sz = nzp + 1;
   constexpr int sc = kDirectCols + 1;
-  const sp<kS, double> S = (sp<kS, double>)dl;   // nzp x 16 (stride 17)
-  const sp<kS, double> Tm = S + nzp * sc;        // nzp x 16
-  const sp<kS, double> Wi = Tm + nzp * sc;       // Vz^-1 (nzp x nzp, stride sz)
-  const sp<kS, double> W = Wi + nzp * sz;        // Vz
-  const auto m = gp(L.dmat);
+  const sp<kS, CT> S = (sp<kS, CT>)(sp<kS, double>)dl;   // nzp x 16 (stride 17)
+  const sp<kS, CT> Tm = S + nzp * sc;        // nzp x 16
+  const sp<kS, CT> Wi = Tm + nzp * sc;       // Vz^-1 (nzp x nzp, stride sz)
+  const sp<kS, CT> W = Wi + nzp * sz;        // Vz
+  const auto m = gp(reinterpret_cast<const CT*>(L.dmat));
   const size_t oz = (size_t)2 * nx * nx + (size_t)2 * ny * ny;
-  const auto fh = gp((const double*)L.dscr);
-  const auto inv = gp(L.dinv);
-  stage3<16>(
-      nzp * nzp, [&](int e) { return m[oz + e]; }, [&](int e, double v) { Wi[(e / nzp) * sz + e % nzp] = v; },
+  const auto fh = gp(reinterpret_cast<const CT*>(L.dscr));
+  const auto inv = gp(reinterpret_cast<const CT*>(L.dinv));
+  stage3<16, CT>(
+      nzp * nzp, [&](int e) { return m[oz + e]; }, [&](int e, CT v) { Wi[(e / nzp) * sz + e % nzp] = v; },
       nzp * nzp, [&](int e) { return m[oz + (size_t)nzp * nzp + e]; },
-      [&](int e, double v) { W[(e / nzp) * sz + e % nzp] = v; }, nzp * kDirectCols,
+      [&](int e, CT v) { W[(e / nzp) * sz + e % nzp] = v; }, nzp * kDirectCols,
       [&](int e) {
         const int k = min(e / kDirectCols, nz - 1);
         return fh[(size_t)k * plane + m0 + e % kDirectCols];
       },
-      [&](int e, double v) { S[(e / kDirectCols) * sc + e % kDirectCols] = e / kDirectCols < nz ? v : 0.0; });
+      [&](int e, CT v) { S[(e / kDirectCols) * sc + e % kDirectCols] = e / kDirectCols < nz ? v : CT(0); });
   __syncthreads();
-  mfma_products(Wi, sz, S, sc, nzp, kDirectCols, nzp, [&](int r, int c, double v) {
-    Tm[r * sc + c] = r < nz ? v * inv[(size_t)r * plane + m0 + c] : 0.0;
+  mfma_products<CT>(Wi, sz, S, sc, nzp, kDirectCols, nzp, [&](int r, int c, CT v) {
+    Tm[r * sc + c] = r < nz ? v * inv[(size_t)r * plane + m0 + c] : CT(0);
   });
   __syncthreads();
-  const auto xo = gp(L.dscr) + (size_t)L.ncell;
-  mfma_products(W, sz, Tm, sc, nzp, kDirectCols, nzp, [&](int r, int c, double v) {
+  const auto xo = gp(reinterpret_cast<CT*>(L.dscr)) + (size_t)L.ncell;
+  mfma_products<CT>(W, sz, Tm, sc, nzp, kDirectCols, nzp, [&](int r, int c, CT v) {
     if (r < nz) xo[(size_t)r * plane + m0 + c] = v;
   });
 }
@@ -2439,11 +2463,12 @@ size_t direct_lds_limit() {
       hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
       v > 0)
     lim = (size_t)v;
-  const void* kernels[] = {reinterpret_cast<const void*>(&k_direct_xy<double, true>),
-                           reinterpret_cast<const void*>(&k_direct_xy<float, true>),
-                           reinterpret_cast<const void*>(&k_direct_xy<double, false>),
-                           reinterpret_cast<const void*>(&k_direct_xy<float, false>),
-                           reinterpret_cast<const void*>(&k_direct_z)};
+  const void* kernels[] = {reinterpret_cast<const void*>(&k_direct_xy<double, double, true>),
+                           reinterpret_cast<const void*>(&k_direct_xy<float, float, true>),
+                           reinterpret_cast<const void*>(&k_direct_xy<double, double, false>),
+                           reinterpret_cast<const void*>(&k_direct_xy<float, float, false>),
+                           reinterpret_cast<const void*>(&k_direct_z<double>),
+                           reinterpret_cast<const void*>(&k_direct_z<float>)};
   bool ok = true;
   for (const void* k : kernels)
     ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) == hipSuccess;
@@ -2454,17 +2479,22 @@ size_t direct_lds_limit() {
   limit = lim;
   return limit;
 }
-size_t direct_lds(int nx, int ny, int nz) {
-  return std::max(direct_lds_xy(nx, ny), direct_lds_z(direct_nzp(nz)));
+size_t direct_lds(int nx, int ny, int nz, bool fp64) {
+  const size_t es = fp64 ? sizeof(double) : sizeof(float);
+  return std::max(direct_lds_xy(nx, ny, es), direct_lds_z(direct_nzp(nz), es));
 }
 void launch_direct(bool fp64, const DevLevel* L, const LaunchList& planes, const LaunchList& slabs,
                    size_t lds, hipStream_t s) {
   const dim3 blk(kBlock);
-  if (fp64) hipLaunchKernelGGL((k_direct_xy<double, true>), dim3(planes.nblocks), blk, lds, s, L, planes);
-  else hipLaunchKernelGGL((k_direct_xy<float, true>), dim3(planes.nblocks), blk, lds, s, L, planes);
-  hipLaunchKernelGGL(k_direct_z, dim3(slabs.nblocks), blk, lds, s, L, slabs);
-  if (fp64) hipLaunchKernelGGL((k_direct_xy<double, false>), dim3(planes.nblocks), blk, lds, s, L, planes);
-  else hipLaunchKernelGGL((k_direct_xy<float, false>), dim3(planes.nblocks), blk, lds, s, L, planes);
+  if (fp64) {
+    hipLaunchKernelGGL((k_direct_xy<double, double, true>), dim3(planes.nblocks), blk, lds, s, L, planes);
+    hipLaunchKernelGGL(k_direct_z<double>, dim3(slabs.nblocks), blk, lds, s, L, slabs);
+    hipLaunchKernelGGL((k_direct_xy<double, double, false>), dim3(planes.nblocks), blk, lds, s, L, planes);
+  } else {
+    hipLaunchKernelGGL((k_direct_xy<float, float, true>), dim3(planes.nblocks), blk, lds, s, L, planes);
+    hipLaunchKernelGGL(k_direct_z<float>, dim3(slabs.nblocks), blk, lds, s, L, slabs);
+    hipLaunchKernelGGL((k_direct_xy<float, float, false>), dim3(planes.nblocks), blk, lds, s, L, planes);
+  }
 }
 void launch_coarse(bool fp64, const DevLevel* L, const LaunchList& ll, hipStream_t s) {
   if (fp64) LAUNCH(k_coarse<double>, ll.nblocks, kBlock, L, ll);

@@ -507,34 +507,43 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     I->tail = (int)hT.size();
     I->tail_lds = off;
   }
-  // separable direct solve of the first level below the finest with at most
-  // kDirectCells cells, when that level lies above the coarse tail (the
-  // tail solves small levels in one launch); the levels below it are not
-  // cycled
+  // separable direct solve of the first level below the finest, above the
+  // coarse tail (the tail solves small levels in one launch), that the
+  // direct solve takes: at most kDirectCells cells, nx and ny multiples of
+  // 16, nz <= kDirectMaxNz, the operands within the LDS; the levels below it
+  // are not cycled.  A mixed solve computes it in FP32 (its tables are
+  // FP32), an FP64 one in FP64.
   if (cfg.coarse_direct != 0) {
-    int cand = -1;
-    for (int l = 1; l < L; ++l)
-      if (I->H.lv[l].ncell() <= kDirectCells) { cand = l; break; }
-    if (cand >= 1 && cand < I->l0) {
+    const bool f64 = I->fp64;
+    const size_t es = f64 ? sizeof(double) : sizeof(float);
+    for (int cand = 1; cand < I->l0; ++cand) {
       const HostLevel& H = I->H.lv[cand];
-      const size_t lds = direct_lds(H.nx, H.ny, H.nz);
+      if (H.ncell() > kDirectCells) continue;
+      const size_t lds = direct_lds(H.nx, H.ny, H.nz, f64);
       DirectTables D;
       std::string derr;
-      if (H.nx % 16 == 0 && H.ny % 16 == 0 && H.nz <= kDirectMaxNz && lds <= direct_lds_limit() &&
-          direct_setup(H, I->H.bc, I->H.singular, &D, &derr)) {
-        double* dm = (double*)I->mem.alloc(D.mats.size() * 8);
-        double* di = (double*)I->mem.alloc(D.inv.size() * 8);
-        double* ds = (double*)I->mem.alloc(2 * (size_t)H.ncell() * 8);
-        if (!dm || !di || !ds) return fail();
-        AMG_HIP(hipMemcpy(dm, D.mats.data(), D.mats.size() * 8, hipMemcpyHostToDevice));
-        AMG_HIP(hipMemcpy(di, D.inv.data(), D.inv.size() * 8, hipMemcpyHostToDevice));
-        DevLevel& d = newL[cand];
-        d.dmat = dm;
-        d.dinv = di;
-        d.dscr = ds;
-        I->ld = cand;
-        I->dlds = lds;
+      if (!(H.nx % 16 == 0 && H.ny % 16 == 0 && H.nz <= kDirectMaxNz && lds <= direct_lds_limit() &&
+            direct_setup(H, I->H.bc, I->H.singular, &D, &derr)))
+        continue;
+      void* dm = I->mem.alloc(D.mats.size() * es);
+      void* di = I->mem.alloc(D.inv.size() * es);
+      void* ds = I->mem.alloc(2 * (size_t)H.ncell() * es);
+      if (!dm || !di || !ds) return fail();
+      if (f64) {
+        AMG_HIP(hipMemcpy(dm, D.mats.data(), D.mats.size() * es, hipMemcpyHostToDevice));
+        AMG_HIP(hipMemcpy(di, D.inv.data(), D.inv.size() * es, hipMemcpyHostToDevice));
+      } else {
+        const std::vector<float> m32(D.mats.begin(), D.mats.end()), i32(D.inv.begin(), D.inv.end());
+        AMG_HIP(hipMemcpy(dm, m32.data(), m32.size() * es, hipMemcpyHostToDevice));
+        AMG_HIP(hipMemcpy(di, i32.data(), i32.size() * es, hipMemcpyHostToDevice));
       }
+      DevLevel& d = newL[cand];
+      d.dmat = (double*)dm;
+      d.dinv = (double*)di;
+      d.dscr = (double*)ds;
+      I->ld = cand;
+      I->dlds = lds;
+      break;
     }
   }
   // projection metrics

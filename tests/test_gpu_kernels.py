@@ -372,23 +372,26 @@ def test_zline_lean_form_full_columns(cuda, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("n,beta,tbc,nlevels,direct", [
-    ((64, 64, 64), 2.0, (0, 1, 0, 1, 0, 0), None, (32, 32, 32)),      # singular, stretched z
-    ((128, 64, 64), 1.0, (2, 2, 2, 2, 2, 2), None, (64, 32, 32)),     # Dirichlet x, y and z
-    ((96, 96, 48), 1.5, (1, 1, 1, 1, 1, 1), None, (48, 48, 24)),      # Neumann everywhere
-    ((256, 64, 256), 0.0, (0, 2, 0, 2, 0, 0), (4, 3, 8), (64, 16, 64)),  # production levels
+    ((64, 64, 64), 2.0, (0, 1, 0, 1, 0, 0), None, ((32, 32, 32),) * 2),     # singular, stretched z
+    ((128, 64, 64), 1.0, (2, 2, 2, 2, 2, 2), None, ((64, 32, 32),) * 2),    # Dirichlet x, y and z
+    ((96, 96, 48), 1.5, (1, 1, 1, 1, 1, 1), None, ((48, 48, 24),) * 2),     # Neumann everywhere
+    ((256, 64, 256), 0.0, (0, 2, 0, 2, 0, 0), (4, 3, 8), ((64, 16, 64),) * 2),   # production levels
+    ((256, 256, 256), 2.0, (0, 1, 0, 1, 0, 0), None, ((64, 64, 64),) * 2),      # the benchmarked levels
 ])
 def test_direct_level_solve(cuda, precision, n, beta, tbc, nlevels, direct):
     """The separable direct solve of the coarse level (transforms with the
     eigenvectors of the x, y and z operators on the matrix cores, scaling by
-    the inverse eigenvalues, transforms back; FP64 arithmetic) against the
-    oracle's restatement (numpy eigh in x and y, Thomas along z, pinned
-    against a sparse LU) on a random right-hand side: 1e-11 in FP64, the
-    FP32 rounding of the result in mixed precision."""
+    the inverse eigenvalues, transforms back) against the oracle's
+    restatement (numpy eigh in x and y, Thomas along z, pinned against a
+    sparse LU) on a random right-hand side.  An FP64 V-cycle's level is
+    solved in FP64 arithmetic, to 1e-11; a mixed solve's FP32 V-cycle's in
+    FP32 arithmetic, to 2e-5."""
     from oracle import ops, vcycle as VC
     from oracle.grid import is_singular
     s, g, levs, fac = _solver(n, beta, tbc, "LJ", precision, rho=True, nlevels=nlevels)
     ld = s.direct_level
-    assert ld >= 1 and s.levels[ld] == direct, (ld, s.levels)
+    want = direct[0] if precision == "fp64" else direct[1]
+    assert ld >= 1 and s.levels[ld] == want, (ld, s.levels)
     lev = levs[ld]
     rng = np.random.default_rng(21)
     f = rng.standard_normal(tuple(reversed(lev["n"])))
@@ -402,5 +405,5 @@ def test_direct_level_solve(cuda, precision, n, beta, tbc, nlevels, direct):
     ref = VC.separable_solve(lev, fd, sing)
     err = relerr(got, ref)
     print(f"direct {direct} {precision}: err {err:.2e}", flush=True)
-    assert err <= (1e-11 if precision == "fp64" else 2e-6), err
+    assert err <= (1e-11 if precision == "fp64" else 2e-5), err
     s.close()
