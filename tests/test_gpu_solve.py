@@ -341,6 +341,37 @@ def test_paired_cycles(cuda, smoother):
     assert abs(info["iters"] - its[1]) <= 1 and len(info["cycles"]) < info["iters"]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("pre0,post0", [(2, 0), (0, 2)])
+def test_finest_level_without_pre_or_post_smoothing(cuda, precision, pre0, post0):
+    """A finest level with no post-smoothing (its prolongation then
+    corrects every cell) or no pre-smoothing (its marching restriction then
+    forms every residual) still converges to the exact solution (1e-9 at
+    relres 1e-12); the coarser levels keep V(2,2)."""
+    torch = cuda
+    from anelastic_solve_amd import PressureSolver, box_grid, default_config, workloads
+    tbc = (0, 1, 0, 1, 0, 0)
+    g = box_grid(128, 128, 128, beta=2.0)
+    U, V, W = workloads.velocity_spectral(g, 3)
+    b = workloads.psource_host(U, V, W, g)
+    # coarsest-first sweep counts: the finest level's is the last entry
+    pre = [2] * 6
+    post = [2] * 6
+    pre[-1], post[-1] = pre0, post0
+    cfg = default_config(smoother="LJ", precision=precision, tbc=tbc, vbc=[-1.0, 0.0, -1.0, 0.0, -1.0, -1.0],
+                         maxcy=100, tol=1e-12, pre=pre, post=post)
+    s = PressureSolver(g, cfg)
+    assert len(s.levels) == 6 and s.levels[0] == (128, 128, 128)
+    x, levs = _direct(g, b, tbc, s.levels)
+    p = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
+    it, _ = s.mg_solver(p, torch.from_numpy(b).cuda())
+    assert s.stats.converged == 1 and s.stats.relres <= 1e-12, (it, s.stats.relres)
+    err = relerr(p.cpu().numpy()[1:-1, 1:-1, 1:-1], x)
+    print(f"{precision} pre0 {pre0} post0 {post0}: {it} cycles, err {err:.2e}", flush=True)
+    assert err <= 1e-9, err
+    s.close()
+
+
 def test_server_allows_device_synchronisation(cuda):
     """The caller's thread may synchronise the whole device while the
     server's worker runs its rounds (the worker captures no graphs, which
