@@ -1692,8 +1692,8 @@ __device__ GateWords item_gate_check(ItemGate* g, const double* out, const doubl
 }
 
 // One thread per item: the gate of a starting solve, from its first defect
-// (the decisions of the host's Context::prepare, with its operations), and
-// the plan of its first cycle.
+// (the decisions the host's replay, Context::absorb_gate, takes with the
+// same operations), and the plan of its first cycle.
 __global__ void k_gate_init(LaunchList ll, const double* __restrict__ out, ItemGate* ig,
                             DevLevel* Lt, int nL, int maxcy, int rule, double tol, int report) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;

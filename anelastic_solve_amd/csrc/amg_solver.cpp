@@ -1004,11 +1004,6 @@ int Context::sync_gates(const std::vector<Instance*>& S, double* bytes) {
   return AMG_OK;
 }
 
-int Context::prepare(const std::vector<Instance*>& S) {
-  int rc = prepare_async(S);
-  if (rc != AMG_OK) return rc;
-  return sync_gates(S, nullptr);
-}
 
 int Context::absorb_gate(Instance* I, const ItemGate& g, double* bytes) {
   if (!I->gate_seen) {
@@ -1135,7 +1130,9 @@ int Context::gated_cycles(Instance* I, double* bytes) {
 int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
   if (S.empty()) return AMG_OK;
   AMG_HIP(hipEventRecord(ev[2], stream));
-  int rc = prepare(S);
+  // the first defect's gates are read with the first cycles' (a solve that
+  // the first defect ends has its cycles return at once on the device)
+  int rc = prepare_async(S);
   if (rc != AMG_OK) return rc;
   AMG_HIP(hipEventRecord(ev[0], stream));
   double bytes = 0.0;

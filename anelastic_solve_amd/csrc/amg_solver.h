@@ -123,9 +123,8 @@ struct Context {
   void rb_launches(const std::vector<Item>& items, bool fp64, int parity, int zg);
   void direct_launches(const std::vector<Item>& planes, const std::vector<Item>& modes, size_t lds,
                        bool fp64);
-  int prepare(const std::vector<Instance*>& S);
-  // the start of a solve (prepare) enqueued only: the gates of S are
-  // initialised on the device, the host reads them later
+  // the start of a solve, enqueued only: the gates of S are initialised on
+  // the device, the host reads them with the first cycles' (sync_gates)
   int prepare_async(const std::vector<Instance*>& S);
   int gate_slots() const;
   // copies the gates to the host, waits, and replays those of S
