@@ -357,7 +357,10 @@ struct GateArgs {
   ItemGate* ig;
 };
 // Clears the state words of the instances of the items of ll.
-void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s);
+// only_stopped: just the instances whose solve has stopped (a reset that may
+// be enqueued before the host knows whether it has).
+void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s,
+                       bool only_stopped = false);
 // Initialises the gates of the items of ll (FP64 finest levels) from the
 // first defect's norms out + t*4 = {max|r|, max|b|, ..}; an item whose
 // solve ends there (NaN, or the residual rule already holds) is stopped.

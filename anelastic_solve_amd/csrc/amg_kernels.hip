@@ -1885,7 +1885,9 @@ __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt
   const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
   const auto vol = gp(L.vol);
   const auto rho = gp(L.rho);
-  const auto x = gfield<const double>((const void*)(which == 0 ? L.u : L.f));
+  // which 0: the current pressure of an FP64 finest entry (its ping-pong
+  // state word pcur says which buffer holds it)
+  const auto x = gfield<const double>((const void*)(which == 0 ? (L.pcur ? L.u2 : L.u) : L.f));
   double v[1] = {0.0};
   // unconditional loads (clamped indices), a chunk of kWsumU cells issued
   // before their uses, as in k_ingest; cells past the item's end add zero
@@ -1926,7 +1928,8 @@ __global__ __launch_bounds__(kBlock) void k_to_fortran(const DevLevel* __restric
   if (e >= L.ncell) return;
   int i, j, k;
   decode(L, e, i, j, k);
-  double x = gfield<const double>((const void*)L.u)[e];
+  // the current pressure (pcur: which buffer of the ping-pong pair holds it)
+  double x = gfield<const double>((const void*)(L.pcur ? L.u2 : L.u))[e];
   if (c && L.nullsp) x -= (c[t] / L.vrsum) * gp(L.rho)[k];
   gp(ptr.p[t])[fidx(i, j, k, L.nx, L.ny)] = x;
 }
@@ -2353,10 +2356,11 @@ inline unsigned nb(size_t n, int bs = kBlock) { return (unsigned)((n + bs - 1) /
 }  // namespace
 
 // Clears the state words of the instances of the items of ll.
-__global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, LaunchList ll) {
+__global__ void k_gate_reset(DevLevel* __restrict__ Lt, int nL, LaunchList ll, int only_stopped) {
   for (int l = threadIdx.x; l < nL; l += blockDim.x) {
     bool mine = false;
     for (int t = 0; t < ll.nitems; ++t) mine = mine || Lt[ll.items[t]].inst == Lt[l].inst;
+    if (only_stopped && !Lt[l].stop) mine = false;
     if (mine) {
       Lt[l].stop = 0;
       Lt[l].warm = 0;
@@ -2594,8 +2598,8 @@ void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* pa
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);
 }
-void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s) {
-  hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(256), 0, s, L, nL, ll);
+void launch_gate_reset(DevLevel* L, int nL, const LaunchList& ll, hipStream_t s, bool only_stopped) {
+  hipLaunchKernelGGL(k_gate_reset, dim3(1), dim3(256), 0, s, L, nL, ll, only_stopped ? 1 : 0);
 }
 void launch_gate_init(const LaunchList& ll, const double* out, ItemGate* ig, DevLevel* L, int nL,
                       int maxcy, int rule, double tol, int report, hipStream_t s) {

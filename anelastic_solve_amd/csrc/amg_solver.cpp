@@ -928,7 +928,7 @@ const double* Context::null_sums(const std::vector<Instance*>& S) {
   std::vector<Item> sums;
   for (Instance* I : S) {
     any = any || I->H.singular;
-    const int li = I->fine[I->cur];
+    const int li = I->fine[0];   // the device's pcur picks the current buffer
     sums.push_back({li, cdiv(hL[li].ncell, (long long)kBlock * kWsumPer)});
   }
   if (!any) return nullptr;
@@ -940,6 +940,16 @@ const double* Context::null_sums(const std::vector<Instance*>& S) {
   return c;
 }
 
+// The state words of the instances of S back to zero (only_stopped: of those
+// whose solve has stopped).
+void Context::reset_gates(const std::vector<Instance*>& S, bool only_stopped) {
+  std::vector<Item> mine;
+  for (Instance* I : S) mine.push_back({I->fine[0], 1});
+  for_lists(mine, [&](const LaunchList& ll, int, int) {
+    launch_gate_reset(dL, (int)hL.size(), ll, stream, only_stopped);
+  });
+}
+
 // The start of a solve for every instance of S, enqueued only: the state
 // words cleared, the Fortran arrays in the chequer layout (b with the
 // boundary fold) with sum V b, the first defect (mixed: also the FP32 defect
@@ -949,7 +959,7 @@ const double* Context::null_sums(const std::vector<Instance*>& S) {
 int Context::prepare_async(const std::vector<Instance*>& S) {
   const int nv = 4;
   PtrList rhs{}, phi{};
-  std::vector<Item> items, mine;
+  std::vector<Item> items;
   for (Instance* I : S) {
     I->cur = 0;
     I->iters = 0;
@@ -959,11 +969,8 @@ int Context::prepare_async(const std::vector<Instance*>& S) {
     I->status = AMG_OK;
     I->gate_seen = false;
     items.push_back({I->fine[0], cdiv(hL[I->fine[0]].ncell, (long long)kBlock * kWsumPer)});
-    mine.push_back({I->fine[0], 1});
   }
-  for_lists(mine, [&](const LaunchList& ll, int, int) {
-    launch_gate_reset(dL, (int)hL.size(), ll, stream);
-  });
+  reset_gates(S, false);
   double* c = d_out + kOutRegion * 3;
   for_lists(items, [&](const LaunchList& ll, int ioff, int boff) {
     for (int t = 0; t < ll.nitems; ++t) {
@@ -1054,9 +1061,11 @@ int Context::absorb_gate(Instance* I, const ItemGate& g, double* bytes) {
 }
 
 int Context::finish(const std::vector<Instance*>& S) {
+  // the first FP64 entry: the device's pcur picks the buffer of the current
+  // pressure, so the copy out needs no host view of the solve
   std::vector<Item> cur;
   for (Instance* I : S) {
-    const int li = I->fine[I->cur];
+    const int li = I->fine[0];
     cur.push_back({li, cdiv(hL[li].ncell, kBlock)});
   }
   // null-space removal, fused into the copy out: p -= (sum V p / sum V rho) rho
@@ -1092,11 +1101,19 @@ int Context::finish(const std::vector<Instance*>& S) {
 // captured once: the launch lists do not depend on the solve's state, which
 // the kernels read from the instance's state words.  When capture is
 // unavailable the caller's loop takes over.
-int Context::gated_cycles(Instance* I, double* bytes) {
+// A batch that should end the solve (it reaches the previous solve's cycle
+// count) is followed, before the host looks, by the solve's tail: the copy
+// out (which reads the device's state, not the host's) and the state reset
+// of a stopped solve.  If the solve has not stopped after all, the reset
+// leaves it alone and the copy out is made again at the end; *tail tells
+// the caller whether it stands.
+int Context::gated_cycles(Instance* I, double* bytes, bool* tail) {
   const std::vector<Instance*> A{I};
+  *tail = false;
   while (!I->done) {
     int batch = I->last_iters > I->dev_cycles ? I->last_iters - I->dev_cycles : 1;
     batch = std::max(1, std::min(batch, std::min(kItemRing, cfg.maxcy - I->dev_cycles)));
+    const bool last = I->last_iters > 0 && I->dev_cycles + batch >= I->last_iters;
     std::vector<int> plan;
     for (int left = batch; left > 0;) {
       const int q = left >= 4 ? 2 : left >= 2 ? 1 : 0;
@@ -1121,8 +1138,16 @@ int Context::gated_cycles(Instance* I, double* bytes) {
       }
     }
     for (int q : plan) AMG_HIP(hipGraphLaunch(gchain[q], stream));
+    if (last) {
+      AMG_HIP(hipEventRecord(ev[1], stream));
+      const int rf = finish(A);
+      if (rf != AMG_OK) return rf;
+      reset_gates(A, true);
+      AMG_HIP(hipEventRecord(ev[3], stream));
+    }
     const int rc = sync_gates(A, bytes);
     if (rc != AMG_OK) return rc;
+    *tail = last && I->done;
   }
   return AMG_OK;
 }
@@ -1136,8 +1161,9 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
   if (rc != AMG_OK) return rc;
   AMG_HIP(hipEventRecord(ev[0], stream));
   double bytes = 0.0;
+  bool tail = false;
   if (S.size() == 1 && cfg.use_graph && graph_ok) {
-    rc = gated_cycles(S[0], &bytes);
+    rc = gated_cycles(S[0], &bytes, &tail);
     if (rc != AMG_OK) return rc;
   }
   // one gated cycle of every running instance at a time
@@ -1155,21 +1181,17 @@ int Context::solve(const std::vector<Instance*>& S, amg_stats* st) {
       set_error("gated cycles: the device did not stop within maxcy cycles");
       return AMG_EHIP;
     }
-  AMG_HIP(hipEventRecord(ev[1], stream));
   for (Instance* I : S) I->last_iters = I->iters;
-  rc = finish(S);
-  if (rc != AMG_OK) return rc;
-  // the state words back to running (kernels launched on the levels later,
-  // e.g. run_kernel, must not find a stop)
-  {
-    std::vector<Item> mine;
-    for (Instance* I : S) mine.push_back({I->fine[0], 1});
-    for_lists(mine, [&](const LaunchList& ll, int, int) {
-      launch_gate_reset(dL, (int)hL.size(), ll, stream);
-    });
+  if (!tail) {
+    AMG_HIP(hipEventRecord(ev[1], stream));
+    rc = finish(S);
+    if (rc != AMG_OK) return rc;
+    // the state words back to running (kernels launched on the levels later,
+    // e.g. run_kernel, must not find a stop)
+    reset_gates(S, false);
+    AMG_HIP(hipEventRecord(ev[3], stream));
+    AMG_HIP(hipStreamSynchronize(stream));
   }
-  AMG_HIP(hipEventRecord(ev[3], stream));
-  AMG_HIP(hipStreamSynchronize(stream));
   AMG_HIP(hipGetLastError());
   int status = AMG_OK;
   for (Instance* I : S) {

@@ -136,7 +136,10 @@ struct Context {
   // host's last look (its gate as copied after a batch or round), adding
   // their bytes; AMG_EHIP when the device and the host disagree
   int absorb_gate(Instance* I, const ItemGate& g, double* bytes);
-  int gated_cycles(Instance* I, double* bytes);
+  // *tail: the solve's copy out and state reset were enqueued behind its
+  // last batch (nothing is left for the caller to do but read the stats)
+  int gated_cycles(Instance* I, double* bytes, bool* tail);
+  void reset_gates(const std::vector<Instance*>& S, bool only_stopped);
   void post_norms(const std::vector<Instance*>& S, int m);
   int run_kernel(Instance* I, int which, int level, int reps, double* bytes);
   int level_field(Instance* I, int level, int field, int dir, double* buf);
