@@ -381,8 +381,12 @@ template <class T> __device__ __forceinline__ T ld_row_nt(sp<kG, const T> row, u
 
 // NBR false: the other colour is zero (zero_guess bit 0, the first
 // half-sweep from u = 0) and is not read.  LEAN: the register-lean form of
-// the segment elimination (below).
-template <class T, int S, int C, bool NBR, bool LEAN>
+// the segment elimination (below).  H: a wave holds two segments of 32
+// columns (lanes 0-31 segment 2w, lanes 32-63 segment 2w+1), so a launch of
+// few column groups spreads over twice the CUs; p and lane (the column
+// within the workgroup's 32) are then per lane, and the z rows are staged
+// in LDS.  Every lane does the same operations in either shape.
+template <class T, int S, int C, bool NBR, bool LEAN, bool H>
 __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int zero_guess, int col0,
                                             int p, int lane, int P, T* __restrict__ lds, bool act) {
   const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
@@ -422,12 +426,14 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   // right-hand-side data -- are unconditional (every address is in bounds,
   // uses are selected) and issued before the first wait: a sweep pays one
   // memory round trip.
-  // FP64 rows do not fit in scalar registers: they are staged in LDS by
-  // lanes 0..S-1 and read back as broadcasts (LDS_COEF).
-  constexpr bool LDS_COEF = sizeof(T) == 8;
+  // FP64 rows do not fit in scalar registers, and with H a wave's two
+  // segments have different rows: they are staged in LDS by lanes 0..S-1
+  // (of each half) and read back as broadcasts (LDS_COEF).
+  constexpr bool LDS_COEF = sizeof(T) == 8 || H;
+  constexpr int CW = H ? 32 : 64;             // columns of a workgroup
   constexpr int SR = LDS_COEF ? 1 : S;
   T A[SR], E[SR], Dz[SR];
-  T* const cA = lds + (sizeof(T) == 8 ? 6 : 12) * P * 64 + p * 3 * S;
+  T* const cA = lds + (sizeof(T) == 8 ? 6 : 12) * P * CW + p * 3 * S;
   T* const cE = cA + S;
   T* const cD = cE + S;
   T za = T(0), ze = T(0), zd = T(0);
@@ -553,11 +559,11 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   // contribute zeros.  Every wave takes part.
   // FP32 double-buffers the state in LDS (one barrier per step); FP64 uses
   // one buffer and two barriers, keeping the block within 64 KiB of LDS.
-  const int sl = p * 64 + lane;
+  const int sl = p * CW + lane;
   T bprev = T(0), tnext = T(0);
   constexpr int kZBuf = sizeof(T) == 8 ? 1 : 2;
   if (Pi > 1) {
-    const int PS = P * 64;
+    const int PS = P * CW;
     T at = v0, ab = vl, ct = w0, cb = wl, rt = y0, rb = yl;
     int buf = 0;
     for (int d = 1; d < Pi; d <<= 1, buf ^= (kZBuf - 1)) {
@@ -566,7 +572,7 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
       W[3 * PS + sl] = cb; W[4 * PS + sl] = rt; W[5 * PS + sl] = rb;
       __syncthreads();
       const bool hm = p - d >= 0, hp = p + d < Pi;
-      const int sm = hm ? (p - d) * 64 + lane : sl, spp = hp ? (p + d) * 64 + lane : sl;
+      const int sm = hm ? (p - d) * CW + lane : sl, spp = hp ? (p + d) * CW + lane : sl;
       // row b of segment p-d, row t of segment p+d
       const T abm = W[PS + sm], cbm = W[3 * PS + sm], rbm = W[5 * PS + sm];
       const T atp = W[spp], ctp = W[2 * PS + spp], rtp = W[4 * PS + spp];
@@ -590,8 +596,8 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
     W[sl] = rt;
     W[PS + sl] = rb;
     __syncthreads();
-    bprev = (p > 0) ? W[PS + (p - 1) * 64 + lane] : T(0);
-    tnext = (p < Pi - 1) ? W[(p + 1) * 64 + lane] : T(0);
+    bprev = (p > 0) ? W[PS + (p - 1) * CW + lane] : T(0);
+    tnext = (p < Pi - 1) ? W[(p + 1) * CW + lane] : T(0);
     __syncthreads();   // the group's LDS may be rewritten by its next call
   }
   // phase 3: back substitution of g - v bprev - w tnext (LEAN: v is
@@ -633,8 +639,10 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
 
 // The LEAN FP32 form is built for 8 waves per SIMD (64 registers), so two
 // 1024-lane blocks share a CU and one's loads overlap the other's solve.
-template <class T, int S, int C, bool NBR, bool LEAN>
-__global__ __launch_bounds__(1024, (LEAN && sizeof(T) == 4) ? 8 : 4) void k_zline(
+// With H a workgroup has at most 8 waves and a CU holds one: up to 256
+// registers.
+template <class T, int S, int C, bool NBR, bool LEAN, bool H>
+__global__ __launch_bounds__(H ? 512 : 1024, H ? 2 : ((LEAN && sizeof(T) == 4) ? 8 : 4)) void k_zline(
     const DevLevel* __restrict__ Lt, LaunchList ll, int parity, int zero_guess) {
   extern __shared__ unsigned char lds_raw[];
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -647,9 +655,16 @@ __global__ __launch_bounds__(1024, (LEAN && sizeof(T) == 4) ? 8 : 4) void k_zlin
   if ((zero_guess & kZgWarm) && !L.warm) return;
   // threadIdx.y is constant across a wave (blockDim.x == 64): say so to the compiler
   const int seg = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  zline_group<T, S, C, NBR, LEAN>(L, parity, zero_flags(L, zero_guess),
-                                  (b - ll.prefix[t]) * 64 + threadIdx.x, seg, threadIdx.x, blockDim.y,
-                                  reinterpret_cast<T*>(lds_raw), true);
+  if constexpr (H) {
+    const int cl = threadIdx.x & 31;
+    zline_group<T, S, C, NBR, LEAN, true>(L, parity, zero_flags(L, zero_guess),
+                                          (b - ll.prefix[t]) * 32 + cl, 2 * seg + (threadIdx.x >> 5),
+                                          cl, 2 * blockDim.y, reinterpret_cast<T*>(lds_raw), true);
+  } else {
+    zline_group<T, S, C, NBR, LEAN, false>(L, parity, zero_flags(L, zero_guess),
+                                           (b - ll.prefix[t]) * 64 + threadIdx.x, seg, threadIdx.x,
+                                           blockDim.y, reinterpret_cast<T*>(lds_raw), true);
+  }
 }
 
 template <class T> struct TailNz;
@@ -2403,40 +2418,62 @@ void launch_rb_point(bool fp64, const DevLevel* L, const LaunchList& ll, int par
   else LAUNCH(k_rb_point<float>, ll.nblocks, kBlock, L, ll, parity, zero_guess);
 }
 static void zline_variant(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
-                          int zero_guess, int segs, hipStream_t s, int form, bool nbr) {
-  const dim3 blk(64, segs);
+                          int zero_guess, int segs, hipStream_t s, int form, bool nbr, bool half) {
+  // half: two segments of 32 columns per wave, segs rounded up to even
+  const int wy = half ? (segs + 1) / 2 : segs, ps = half ? 2 * wy : segs, cw = half ? 32 : 64;
+  const dim3 blk(64, wy);
   // reduced-system state (two buffers in FP32, one in FP64) + staged rows
-  const size_t lds = (size_t)((fp64 ? 6 : 12) * 64 + 3 * kZSeg) * segs * (fp64 ? 8 : 4);
-#define ZL(T, CH, NBR, LEAN)                                                                      \
-  hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR, LEAN>), dim3(ll.nblocks), blk, lds, s, L, ll, parity, \
-                     zero_guess)
+  const size_t lds = (size_t)((fp64 ? 6 : 12) * cw + 3 * kZSeg) * ps * (fp64 ? 8 : 4);
+#define ZL(T, CH, NBR, LEAN, H)                                                                   \
+  hipLaunchKernelGGL((k_zline<T, kZSeg, CH, NBR, LEAN, H>), dim3(ll.nblocks), blk, lds, s, L, ll, \
+                     parity, zero_guess)
   // FP64 always takes the lean form (the short one does not fit its
   // registers).  FP32 takes it when there are more blocks than CUs, so that
   // blocks can pair up; a smaller launch is latency-bound and takes the
-  // shorter dependent chains of the short form.
+  // shorter dependent chains of the short form (and, with half, twice the
+  // workgroups).
   if (fp64) {
-    if (nbr) ZL(double, kZChunk64, true, true);
-    else ZL(double, kZChunk64, false, true);
+    if (half) {
+      if (nbr) ZL(double, kZChunk64, true, false, true);
+      else ZL(double, kZChunk64, false, false, true);
+    } else {
+      if (nbr) ZL(double, kZChunk64, true, true, false);
+      else ZL(double, kZChunk64, false, true, false);
+    }
+  } else if (half) {
+    if (nbr) ZL(float, kZChunk32, true, false, true);
+    else ZL(float, kZChunk32, false, false, true);
   } else if (form == 2 || (form == 0 && ll.nblocks > device_cus())) {
-    if (nbr) ZL(float, kZChunk32, true, true);
-    else ZL(float, kZChunk32, false, true);
+    if (nbr) ZL(float, kZChunk32, true, true, false);
+    else ZL(float, kZChunk32, false, true, false);
   } else {
-    if (nbr) ZL(float, kZChunk32, true, false);
-    else ZL(float, kZChunk32, false, false);
+    if (nbr) ZL(float, kZChunk32, true, false, false);
+    else ZL(float, kZChunk32, false, false, false);
   }
 #undef ZL
 }
 void launch_zline(bool fp64, const DevLevel* L, const LaunchList& ll, int parity, int zero_guess,
                   int segs, hipStream_t s, int form) {
+  // A launch of at most half as many column groups as the device has CUs
+  // (and columns of two segments or more) runs 32 columns per workgroup:
+  // every item's block range doubles.  Forced forms keep 64.
+  const bool half = form == 0 && segs >= 2 && 2 * ll.nblocks <= device_cus();
+  LaunchList hl;
+  if (half) {
+    hl = ll;
+    for (int t = 0; t <= ll.nitems; ++t) hl.prefix[t] = 2 * ll.prefix[t];
+    hl.nblocks = 2 * ll.nblocks;
+  }
+  const LaunchList& l2 = half ? hl : ll;
   // kZgDyn with a zero other colour: the sweep from zero, then the ordinary
   // sweep for the instances whose cycle continues the iterate; each kernel
   // returns at once for the instances of the other
   if ((zero_guess & kZgDyn) && (zero_guess & 1)) {
-    zline_variant(fp64, L, ll, parity, zero_guess, segs, s, form, false);
-    zline_variant(fp64, L, ll, parity, kZgWarm, segs, s, form, true);
+    zline_variant(fp64, L, l2, parity, zero_guess, segs, s, form, false, half);
+    zline_variant(fp64, L, l2, parity, kZgWarm, segs, s, form, true, half);
     return;
   }
-  zline_variant(fp64, L, ll, parity, zero_guess, segs, s, form, !(zero_guess & 1));
+  zline_variant(fp64, L, l2, parity, zero_guess, segs, s, form, !(zero_guess & 1), half);
 }
 void launch_zline_col(bool fp64, const DevLevel* L, const LaunchList& ll, int parity,
                       int zero_guess, hipStream_t s) {

@@ -370,6 +370,49 @@ def test_zline_lean_form_full_columns(cuda, precision):
     s.close()
 
 
+@pytest.mark.parametrize("n,tbc", [((128, 128, 128), (0, 1, 0, 1, 0, 0)),
+                                   ((48, 40, 40), (0, 0, 0, 0, 0, 0))])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_zline_half_wave_groups(cuda, precision, n, tbc):
+    """A launch of at most half as many column groups as the device has CUs
+    runs 32 columns per workgroup, two segments per wave: the 128^3 level of
+    the benchmarked hierarchy (8 full segments per column) and a ragged level
+    (40-cell columns: 3 segments, rounded up to 4; 960 columns per colour, so
+    the last group is partial), periodic z included.  Against the oracle's
+    Thomas solve per column, and bitwise against the forced forms, which
+    keep 64 columns per workgroup."""
+    from oracle import vcycle as VC
+    dt = np.float64 if precision == "fp64" else np.float32
+    nx, ny, nz = n
+    s, g, levs, fac = _solver(n, 1.5, tbc, "LJ", precision)
+    lev = levs[0]
+    rng = np.random.default_rng(17)
+    u = rng.standard_normal((nz, ny, nx)).astype(dt)
+    f = rng.standard_normal((nz, ny, nx)).astype(dt)
+    for which, parity in ((0, 0), (4, 1), (6, 0)):
+        s.level_field(0, 0, u)
+        s.level_field(0, 1, f)
+        s.run_kernel(which, 0, 1)
+        got = s.level_field(0, 0)
+        ref = np.zeros_like(u) if which == 6 else u.copy()
+        VC.zline_half_sweep(ref, f, lev, parity, dt)
+        gm, rm = got, ref
+        if which == 6:
+            m = VC.colour_mask(u.shape, parity, True)
+            gm, rm = got[m], ref[m]
+        err = relerr(gm, rm)
+        print(f"{n} {precision} kernel {which} parity {parity}: err {err:.2e}", flush=True)
+        assert err < _tol(precision, "LJ"), (which, parity, err)
+        if which == 6:
+            continue
+        forced = (12 + parity,) if precision == "fp64" else (10 + parity, 12 + parity)
+        for w in forced:
+            s.level_field(0, 0, u)
+            s.run_kernel(w, 0, 1)
+            assert np.array_equal(s.level_field(0, 0), got), (w, parity)
+    s.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("n,beta,tbc,nlevels,direct", [
     ((64, 64, 64), 2.0, (0, 1, 0, 1, 0, 0), None, ((32, 32, 32),) * 2),     # singular, stretched z
