@@ -46,6 +46,9 @@ struct DevLevel {
   int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
   double vsum;               // finest: sum of cell volumes
   double vrsum;              // finest: sum of V*rho
+  double bshift;             // mixed FP64 finest: the compatibility shift sum V b / sum V, which
+                             // the outer step subtracts from b as it loads it (a state word:
+                             // written by the first defect)
   const double* co64;        // coefficient table, FP64
   const float* co32;         // coefficient table, FP32
   const double* vol;         // [dxc(nx) | dyc(ny) | dzc(nz)] interior widths
@@ -311,9 +314,8 @@ size_t tail_lds_limit();
 // launch_finalize folds them per item into out[item*ostride + v].
 // residual_max:     {max|b - A u|, max|b|}
 // change_max:       {max|u - u2|, max|u|}        (u2 holds the previous iterate)
-// defect32:         r = b - A u -> FP32 f of entry 'link'; {max|r|, max|b|}
-// correct_defect32: u2 = u + e (e = FP32 u of 'link'); r = b - A u2 -> FP32 f;
-//                   {max|r|, max|e|, max|u2|}
+// correct_defect32: u2 = u + e (e = FP32 u of 'link'); r = b' - A u2 -> FP32 f
+//                   of 'link', with b' = b - bshift; {max|r|, max|e|, max|u2|}
 constexpr int kFinePer = 8;
 // The FP64 finest-level stencil kernels (residual, defect, correct+defect)
 // march in z over slabs of kMarchK planes: blocks per item =
@@ -327,11 +329,10 @@ inline long long march_blocks(int hx, int ny, int nz) {
 }
 void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
 void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
-void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s);
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
                              hipStream_t s);
 void launch_zero(double* a, int n, hipStream_t s);
-// weighted sums fold kWsumPer cells per thread: blocks per item = cdiv(ncell, kBlock*kWsumPer)
+// the ingest folds kWsumPer cells per thread: blocks per item = cdiv(ncell, kBlock*kWsumPer)
 constexpr int kWsumPer = 16;
 
 // Division by an invariant d >= 1 of any 32-bit n (Granlund and Montgomery,
@@ -372,14 +373,17 @@ void launch_finalize(const LaunchList& ll, const double* partial, int nvals, int
 // Conversions between the reference's ghosted Fortran arrays (one pointer
 // per item in 'ptr') and the FP64 finest level.
 // from_fortran: dst = (which==0 ? u : f) = src interior (+ bcfold if add_fold)
-// wsum:         {sum V x} (x = u or f), or {sum V rho} when which == 2
+// wsum:         {sum V x} (x = u or f), or {sum V rho} when which == 2;
+//               march_blocks per item (z slabs of kMarchK planes)
 // to_fortran:   dst interior = u, or u - (c[item]/vrsum) rho(z) for singular
 //               items when c is given (c: the sums of launch_wsum)
 // ingest:       f = rhs interior + bcfold, u = phi interior, and one partial
-//               of sum V*f per block (kWsumPer cells per thread, as wsum)
-// first_defect: the first defect of a solve (march_blocks per item), as
-//               residual_max (fp64) / defect32 (mixed), after f -= c[item]/vsum
-//               in place for singular items (c: the sums of launch_ingest)
+//               of sum V*f per block (kWsumPer cells per thread)
+// first_defect: the first defect of a solve (march_blocks per item): FP64
+//               {max|b'-Au|, max|b'|} with b' = f - c[item]/vsum written back
+//               in place for singular items (c: the sums of launch_ingest);
+//               mixed the same norms and the FP32 defect b' - Au, with f kept
+//               and the shift recorded in the item's bshift
 // copy_u:       u2 = u
 void launch_from_fortran(const DevLevel* L, const LaunchList& ll, const PtrList& ptr,
                          int which, int add_fold, hipStream_t s);

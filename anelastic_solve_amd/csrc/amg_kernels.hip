@@ -1507,9 +1507,10 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     const sp<kG, const double> ur = u + pb;
     q.xm = ld_row<double>(ur, dxm); q.xp = ld_row<double>(ur, dxp);
     q.ym = ld_row<double>(ur, dym); q.yp = ld_row<double>(ur, dyp);
-    // cs: the compatibility shift of b (first defect only).  The mixed step
-    // reads b once per cell and nothing else reads it during the step: a
-    // non-temporal load (it was measured slower in the FP64 residual)
+    // cs: the compatibility shift of b, subtracted as b is loaded (the
+    // first defect; the mixed step, which keeps b as ingested).  The mixed
+    // step reads b once per cell and nothing else reads it during the step:
+    // a non-temporal load (it was measured slower in the FP64 residual)
     const sp<kG, const double> fr = (sp<kG, const double>)(f + pb);
     q.fb = (MIXED ? ld_row_nt<double>(fr, dc) : ld_row<double>(fr, dc)) - cs;
     q.pu = ld_row<double>(u + kp * plane, dc);
@@ -1542,8 +1543,9 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
     s += cz[nz + k] * vp;
     const double diag = (cz[2 * nz + k] + dxy_y) + dxy_x;
     const double res = cur.fb - (s + diag * vc);
-    // the shifted b and the FP32 defect are read by later kernels only
-    if constexpr (SHIFT) st_row_nt<double>(f + pbk, dc, cur.fb);
+    // the shifted b (FP64: the V-cycle's finest right-hand side) and the
+    // FP32 defect are read by later kernels only
+    if constexpr (SHIFT && MODE == kResMax) st_row_nt<double>(f + pbk, dc, cur.fb);
     if constexpr (MODE == kCorrDefect32) {
       // the new p is read in the next cycle and the FP32 defect by the next
       // V-cycle's first sweep, both after many more bytes than the L2 holds
@@ -1594,8 +1596,12 @@ __device__ __forceinline__ void fine_march(sp<kG, const double> __restrict__ u,
   }
 }
 
+// SHIFT: the first defect of a solve, which takes the compatibility shift
+// of a singular instance from csum (sum V b per item); FP64 writes the
+// shifted b back, mixed keeps b and records the shift in the instance's
+// bshift word, which its outer steps subtract as they load b.
 template <int MODE, bool SHIFT>
-__global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt, LaunchList ll,
+__global__ __launch_bounds__(kBlock) void k_fine(DevLevel* __restrict__ Lt, LaunchList ll,
                                                  const double* __restrict__ csum,
                                                  double* __restrict__ partial) {
   constexpr int NV = MODE == kCorrDefect32 ? 3 : 2;
@@ -1629,7 +1635,8 @@ __global__ __launch_bounds__(kBlock) void k_fine(const DevLevel* __restrict__ Lt
     const auto co = gp(L.co64);
     const auto cy = co + 3 * nx;
     const auto cz = cy + 3 * ny;
-    const double cs = (SHIFT && L.nullsp) ? csum[t] / L.vsum : 0.0;
+    const double cs = MODE == kCorrDefect32 ? L.bshift : (SHIFT && L.nullsp) ? csum[t] / L.vsum : 0.0;
+    if (SHIFT && MODE == kDefect32 && lb == 0 && threadIdx.x == 0) Lt[ll.items[t]].bshift = cs;
     fine_march<MODE, NV, SHIFT>(gfield<const double>(pin),
                          gfield<const float>((const void*)S.u), gfield<double>(L.f), cs,
                          gfield<double>(pout),
@@ -1647,16 +1654,58 @@ struct GateWords {
 };
 
 // Writes w on the levels of instance inst among entries l0, l0+step, ...
-// of the table (the threads of a block split the table between them).
+// of the table (the threads of a block split the table between them).  The
+// entries' instance numbers are read eight at a time before any write, so
+// a walk pays a memory round trip per eight entries, not per entry.
 __device__ void set_words(DevLevel* Lt, int nL, int inst, GateWords w, int l0, int step) {
-  for (int l = l0; l < nL; l += step) {
-    DevLevel& d = Lt[l];
-    if (d.inst != inst) continue;
-    if (w.stop >= 0) d.stop = w.stop;
-    if (w.warm >= 0) d.warm = w.warm;
-    if (w.skipc >= 0) d.skipc = w.skipc;
-    if (w.pflip) d.pcur ^= 1;
+  constexpr int U = 8;
+  for (int lb = l0; lb < nL; lb += U * step) {
+    int iv[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) iv[j] = lb + j * step < nL ? Lt[lb + j * step].inst : -1;
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      if (iv[j] != inst) continue;
+      DevLevel& d = Lt[lb + j * step];
+      if (w.stop >= 0) d.stop = w.stop;
+      if (w.warm >= 0) d.warm = w.warm;
+      if (w.skipc >= 0) d.skipc = w.skipc;
+      if (w.pflip) d.pcur ^= 1;
+    }
   }
+}
+
+// What a gate check reads besides the norms of its own fold: the gate, the
+// instance's state words and the norms in memory.  k_finalize loads it
+// before it folds, so these round trips overlap the fold's.
+struct GatePre {
+  double o[4], oc[2];
+  double bnorm, tol, rprev;
+  int gstop, cycles, n, maxcy, rule, fp64, report, gpair, skipc, warm;
+  int gslot, inst;
+};
+__device__ __forceinline__ GatePre gate_pre(const ItemGate* g, const double* out, const double* outc,
+                                            const DevLevel& L) {
+  GatePre P;
+  for (int q = 0; q < 4; ++q) P.o[q] = out[q];
+  P.oc[0] = outc[0];
+  P.oc[1] = outc[1];
+  P.gstop = g->stop;
+  P.cycles = g->cycles;
+  P.n = g->count;
+  P.maxcy = g->maxcy;
+  P.rule = g->rule;
+  P.fp64 = g->fp64;
+  P.report = g->report;
+  P.gpair = g->pair;
+  P.bnorm = g->bnorm;
+  P.tol = g->tol;
+  P.rprev = g->rprev;
+  P.skipc = L.skipc;
+  P.warm = L.warm;
+  P.gslot = L.gslot;
+  P.inst = L.inst;
+  return P;
 }
 
 // End of a gated cycle of one instance (one thread; L its item's level,
@@ -1666,18 +1715,15 @@ __device__ void set_words(DevLevel* Lt, int nL, int inst, GateWords w, int l0, i
 // same double operations, so the host's replay reaches the same decision;
 // then the next cycle is planned (paired or not).  Returns the state words
 // the instance's levels take.
-__device__ GateWords item_gate_check(ItemGate* g, const double* out, const double* outc,
-                                     const DevLevel& L) {
-  // every load first (the stores below may alias them as far as the
-  // compiler knows, which would serialise a round trip per load)
-  double o[4], oc[2];
-  for (int q = 0; q < 4; ++q) o[q] = out[q];
-  oc[0] = outc[0];
-  oc[1] = outc[1];
-  const int gstop = g->stop, cycles = g->cycles + 1, n = g->count, maxcy = g->maxcy;
-  const int rule = g->rule, fp64 = g->fp64, report = g->report, gpair = g->pair;
-  const double bnorm = g->bnorm, tol = g->tol, rprev = g->rprev;
-  const int skipc = L.skipc, warm = L.warm;
+// P holds what it reads (gate_pre, with the norms of the calling fold put
+// in); only stores remain here.
+__device__ GateWords item_gate_check(ItemGate* g, const GatePre& P) {
+  const double* o = P.o;
+  const double* oc = P.oc;
+  const int gstop = P.gstop, cycles = P.cycles + 1, n = P.n, maxcy = P.maxcy;
+  const int rule = P.rule, fp64 = P.fp64, report = P.report, gpair = P.gpair;
+  const double bnorm = P.bnorm, tol = P.tol, rprev = P.rprev;
+  const int skipc = P.skipc, warm = P.warm;
   if (gstop) return {-1, -1, -1, 0};
   g->cycles = cycles;
   if (skipc) return {-1, 1, 0, 0};
@@ -1748,6 +1794,17 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
   const int t = blockIdx.x;
   const int b0 = ll.prefix[t], b1 = ll.prefix[t + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the gate's inputs, loaded before the fold (thread 0), and the instance
+  // number of every thread's first table entry (never written on the device)
+  GatePre gpre;
+  int inst0 = -1;
+  if (ga.ig) {
+    if (threadIdx.x < ga.nL) inst0 = ga.Lt[threadIdx.x].inst;
+    if (threadIdx.x == 0) {
+      const DevLevel& L = ga.Lt[ll.items[t]];
+      gpre = gate_pre(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, L);
+    }
+  }
   // one pass over the partials for every value (nvals <= 4), kFinU
   // partials per thread and round: their loads (clamped into the item, so
   // unconditional) are issued before they are folded, in order
@@ -1770,7 +1827,10 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
       }
     }
   }
-  for (int q = 0; q < nvals; ++q) {
+  double fr[4] = {0.0, 0.0, 0.0, 0.0};   // thread 0: the folded values
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= nvals) break;
     double a = acc[q];
     a = op_sum ? wave_sum(a) : wave_max(a);
     if (lane == 0) sm[w] = a;
@@ -1779,6 +1839,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
       double r = sm[0];
       for (int i = 1; i < kFinBlock / 64; ++i) r = op_sum ? r + sm[i] : nmax(r, sm[i]);
       out[(size_t)t * ostride + q] = r;
+      fr[q] = r;
     }
     __syncthreads();
   }
@@ -1786,12 +1847,34 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(LaunchList ll, const dou
     __shared__ GateWords gw;
     __shared__ int ginst;
     if (threadIdx.x == 0) {
-      const DevLevel& L = ga.Lt[ll.items[t]];
-      gw = item_gate_check(ga.ig + L.gslot, ga.out + (size_t)t * 4, ga.outc + (size_t)t * 2, L);
-      ginst = L.inst;
+      // the gate's norms that this fold wrote: its values, not the loads
+      // made before it
+      const double* mine = out + (size_t)t * ostride;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q >= nvals) break;
+        for (int r = 0; r < 4; ++r)
+          if (ga.out + (size_t)t * 4 + r == mine + q) gpre.o[r] = fr[q];
+        for (int r = 0; r < 2; ++r)
+          if (ga.outc + (size_t)t * 2 + r == mine + q) gpre.oc[r] = fr[q];
+      }
+      // (the descriptor's words as loaded before the fold: the stores to
+      // out may alias the table as far as the compiler knows)
+      gw = item_gate_check(ga.ig + gpre.gslot, gpre);
+      ginst = gpre.inst;
     }
     __syncthreads();
-    set_words(ga.Lt, ga.nL, ginst, gw, threadIdx.x, kFinBlock);
+    // the first entry of each thread from its prefetched instance number
+    const GateWords wv = gw;
+    const int gi = ginst;
+    if (threadIdx.x < ga.nL && inst0 == gi) {
+      DevLevel& d = ga.Lt[threadIdx.x];
+      if (wv.stop >= 0) d.stop = wv.stop;
+      if (wv.warm >= 0) d.warm = wv.warm;
+      if (wv.skipc >= 0) d.skipc = wv.skipc;
+      if (wv.pflip) d.pcur ^= 1;
+    }
+    set_words(ga.Lt, ga.nL, gi, wv, threadIdx.x + kFinBlock, kFinBlock);
   }
 }
 
@@ -1890,38 +1973,44 @@ __global__ __launch_bounds__(kBlock) void k_ingest(const DevLevel* __restrict__ 
   block_reduce_store<1, true>(v, partial + b);
 }
 
-// Weighted sums: each thread folds kWsumPer cells (in a fixed order), one
-// partial per block.
+// Weighted sums sum V x (V = dx_i dy_j dz_k), marching in z like the
+// finest-level stencil kernels (march_blocks per item): a thread owns
+// position col of a plane and kMarchK planes, sums dz_k x_k over them in
+// order and weights that sum by dx_i dy_j; one partial per block.  The dz_k
+// are wave-uniform (scalar loads), so a cell costs one vector load.
 __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt, LaunchList ll,
                                                  int which, double* __restrict__ partial) {
   const int b = xcd_swizzle(blockIdx.x, gridDim.x);
   const int t = find_item(ll.prefix, ll.nitems, b);
   const DevLevel& L = Lt[ll.items[t]];
-  const int e0 = (b - ll.prefix[t]) * kBlock * kWsumPer + threadIdx.x;
+  const int hx = L.hx, ny = L.ny, nz = L.nz, nx = L.nx;
+  const int plane = 2 * hx * ny;
+  const int ncb = (plane + kBlock - 1) / kBlock;
+  const int lb = b - ll.prefix[t];
+  const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
+  const int k0 = bk * kMarchK, k1 = min(k0 + kMarchK, nz);
   const auto vol = gp(L.vol);
   const auto rho = gp(L.rho);
   // which 0: the current pressure of an FP64 finest entry (its ping-pong
-  // state word pcur says which buffer holds it)
+  // state word pcur says which buffer holds it); 1: f; 2: rho(z)
   const auto x = gfield<const double>((const void*)(which == 0 ? (L.pcur ? L.u2 : L.u) : L.f));
   double v[1] = {0.0};
-  // unconditional loads (clamped indices), a chunk of kWsumU cells issued
-  // before their uses, as in k_ingest; cells past the item's end add zero
-  constexpr int kWsumU = 8;
+  if (col < plane) {
+    const int m = col % hx, row = col / hx, j = row >> 1;
+    const int i = 2 * m + (((row & 1) + j) & 1);
+    const auto vz = vol + nx + ny;
+    double s = 0.0;
+    if (k1 - k0 == kMarchK) {
+      double xv[kMarchK];
 #pragma unroll
-  for (int s0 = 0; s0 < kWsumPer; s0 += kWsumU) {
-    if (s0 > 0) __builtin_amdgcn_sched_barrier(0);
-    double w[kWsumU], xv[kWsumU];
+      for (int q = 0; q < kMarchK; ++q)
+        xv[q] = which == 2 ? rho[k0 + q] : x[(size_t)(k0 + q) * plane + col];
 #pragma unroll
-    for (int q = 0; q < kWsumU; ++q) {
-      const int e = e0 + (s0 + q) * kBlock;
-      const int ec = e < L.ncell ? e : L.ncell - 1;
-      int i, j, k;
-      decode(L, ec, i, j, k);
-      w[q] = vol[L.nx + L.ny + k] * vol[L.nx + j] * vol[i];
-      xv[q] = (which == 2) ? rho[k] : x[ec];
+      for (int q = 0; q < kMarchK; ++q) s += vz[k0 + q] * xv[q];
+    } else {
+      for (int k = k0; k < k1; ++k) s += vz[k] * (which == 2 ? rho[k] : x[(size_t)k * plane + col]);
     }
-#pragma unroll
-    for (int q = 0; q < kWsumU; ++q) v[0] += (e0 + (s0 + q) * kBlock < L.ncell) ? w[q] * xv[q] : 0.0;
+    v[0] = (vol[i] * vol[nx + j]) * s;
   }
   block_reduce_store<1, true>(v, partial + b);
 }
@@ -2610,13 +2699,15 @@ size_t tail_lds_limit() {
   limit = lim;
   return limit;
 }
+// (k_fine takes the table writable: the mixed first defect records bshift)
 void launch_residual_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH((k_fine<kResMax, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
+  LAUNCH((k_fine<kResMax, false>), ll.nblocks, kBlock, const_cast<DevLevel*>(L), ll, nullptr, partial);
 }
 void launch_first_defect(bool fp64, const DevLevel* L, const LaunchList& ll, const double* c,
                          double* partial, hipStream_t s) {
-  if (fp64) LAUNCH((k_fine<kResMax, true>), ll.nblocks, kBlock, L, ll, c, partial);
-  else LAUNCH((k_fine<kDefect32, true>), ll.nblocks, kBlock, L, ll, c, partial);
+  DevLevel* W = const_cast<DevLevel*>(L);
+  if (fp64) LAUNCH((k_fine<kResMax, true>), ll.nblocks, kBlock, W, ll, c, partial);
+  else LAUNCH((k_fine<kDefect32, true>), ll.nblocks, kBlock, W, ll, c, partial);
 }
 void launch_ingest(const DevLevel* L, const LaunchList& ll, const PtrList& rhs, const PtrList& phi,
                    double* partial, hipStream_t s) {
@@ -2625,12 +2716,10 @@ void launch_ingest(const DevLevel* L, const LaunchList& ll, const PtrList& rhs, 
 void launch_change_max(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
   LAUNCH(k_change_max, ll.nblocks, kBlock, L, ll, partial);
 }
-void launch_defect32(const DevLevel* L, const LaunchList& ll, double* partial, hipStream_t s) {
-  LAUNCH((k_fine<kDefect32, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
-}
 void launch_correct_defect32(const DevLevel* L, const LaunchList& ll, double* partial,
                              hipStream_t s) {
-  LAUNCH((k_fine<kCorrDefect32, false>), ll.nblocks, kBlock, L, ll, nullptr, partial);
+  LAUNCH((k_fine<kCorrDefect32, false>), ll.nblocks, kBlock, const_cast<DevLevel*>(L), ll, nullptr,
+         partial);
 }
 void launch_zero(double* a, int n, hipStream_t s) {
   LAUNCH(k_zero, nb((size_t)n), kBlock, a, n);

@@ -166,6 +166,7 @@ int Context::pull_state() {
     hL[l].warm = dev[l].warm;
     hL[l].skipc = dev[l].skipc;
     hL[l].pcur = dev[l].pcur;
+    hL[l].bshift = dev[l].bshift;
   }
   return AMG_OK;
 }
@@ -929,7 +930,7 @@ const double* Context::null_sums(const std::vector<Instance*>& S) {
   for (Instance* I : S) {
     any = any || I->H.singular;
     const int li = I->fine[0];   // the device's pcur picks the current buffer
-    sums.push_back({li, cdiv(hL[li].ncell, (long long)kBlock * kWsumPer)});
+    sums.push_back({li, (int)march_blocks(hL[li].hx, hL[li].ny, hL[li].nz)});
   }
   if (!any) return nullptr;
   double* c = d_out + kOutRegion * 3;
