@@ -397,10 +397,16 @@ void launch_first_defect(bool fp64, const DevLevel* L, const LaunchList& ll, con
                          double* partial, hipStream_t s);
 void launch_copy_u(const DevLevel* L, const LaunchList& ll, const int* active, hipStream_t s);
 
-// Ghost layers of a ghosted FP64 array (the serial path of mgMK/mg_bdry.F90).
+// Ghost layers of a ghosted FP64 array (the serial path of mgMK/mg_bdry.F90):
+// the periodic wrap of one direction over full planes, or every ghost cell
+// at once (the wraps, then the faces' rules of bc, in one pass).
+struct GhostBC {
+  int type[7];      // faces 1..6: 0 periodic (no rule), 1 Neumann, 2 Dirichlet
+  double v[7];      // the face value
+  double vd[7];     // v * dxc of the face's edge cell (the Neumann offset)
+};
 void launch_ghost_wrap(double* a, int nx, int ny, int nz, int dir, hipStream_t s);
-void launch_ghost_face(double* a, int nx, int ny, int nz, int face, int type, double v,
-                       double dedge, hipStream_t s);
+void launch_ghost_fill(double* a, int nx, int ny, int nz, const GhostBC& bc, hipStream_t s);
 
 // Projection step (reference layout, FP64).
 void launch_psource(const double* U, const double* V, const double* W, double* SRC,

@@ -2084,28 +2084,63 @@ __global__ void k_ghost_wrap(double* __restrict__ a, int nx, int ny, int nz, int
   }
 }
 
-// face: 1 X2max, 2 X3min, 3 X2min, 4 X3max, 5 X1min, 6 X1max; type 1 Neumann, 2 Dirichlet.
-__global__ void k_ghost_face(double* __restrict__ a, int nx, int ny, int nz, int face, int type,
-                             double v, double dedge) {
+// Every ghost cell of a ghosted field in one pass, with the values of the
+// serial sequence (mgMK/mg_bdry.F90:71-190): the three periodic wraps over
+// full planes, then the non-periodic faces in the order 1 X2max, 2 X3min,
+// 3 X2min, 4 X3max, 5 X1min, 6 X1max, each from the layer inside it as the
+// earlier passes left it.  A wrap only copies, so after the wraps a ghost
+// cell holds the interior value at its coordinates wrapped one by one.  A
+// face rule reads the cell one step inside its face, which the last earlier
+// face whose layer holds that cell wrote, and so on: walking the faces from
+// the last down and stepping inside each face whose layer holds the current
+// cell gives that chain, whose rules are then applied forward to the
+// wrapped interior value.  Interior cells are only read, and a cell on an
+// edge or corner is computed alike by every thread that owns it.
+// GhostBC: per face 1..6, type (0 periodic: no rule, 1 Neumann, 2
+// Dirichlet), the value v and v*dxc(edge cell) (the Neumann offset, a
+// product made on the host so the rule is one rounding, x -/+ vd).
+__global__ __launch_bounds__(kBlock) void k_ghost_fill(double* __restrict__ a, int nx, int ny, int nz,
+                                                       GhostBC bc) {
   const int sx = nx + 2, sy = ny + 2, sz = nz + 2;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  auto at = [&](int i, int j, int k) -> double& { return a[((size_t)k * sy + j) * sx + i]; };
-  if (face == 1 || face == 3) {
-    if (e >= sx * sz) return;
-    const int i = e % sx, k = e / sx;
-    if (face == 1) at(i, ny + 1, k) = (type == 1) ? at(i, ny, k) + v * dedge : v;
-    else at(i, 0, k) = (type == 1) ? at(i, 1, k) - v * dedge : v;
-  } else if (face == 2 || face == 4) {
-    if (e >= sx * sy) return;
-    const int i = e % sx, j = e / sx;
-    if (face == 4) at(i, j, nz + 1) = (type == 1) ? at(i, j, nz) + v * dedge : v;
-    else at(i, j, 0) = (type == 1) ? at(i, j, 1) - v * dedge : v;
+  const long long nxf = 2LL * sy * sz, nyf = 2LL * sx * sz, nzf = 2LL * sx * sy;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  if (e < nxf) {
+    const long long r = e >> 1;
+    i = (e & 1) ? nx + 1 : 0; j = (int)(r % sy); k = (int)(r / sy);
+  } else if (e < nxf + nyf) {
+    const long long r = (e - nxf) >> 1;
+    j = ((e - nxf) & 1) ? ny + 1 : 0; i = (int)(r % sx); k = (int)(r / sx);
+  } else if (e < nxf + nyf + nzf) {
+    const long long r = (e - nxf - nyf) >> 1;
+    k = ((e - nxf - nyf) & 1) ? nz + 1 : 0; i = (int)(r % sx); j = (int)(r / sx);
   } else {
-    if (e >= sy * sz) return;
-    const int j = e % sy, k = e / sy;
-    if (face == 6) at(nx + 1, j, k) = (type == 1) ? at(nx, j, k) + v * dedge : v;
-    else at(0, j, k) = (type == 1) ? at(1, j, k) - v * dedge : v;
+    return;
   }
+  int chain[6], nc = 0;
+  int ci = i, cj = j, ck = k;
+  for (int f = 6; f >= 1; --f) {
+    if (!bc.type[f]) continue;
+    bool in = false;
+    switch (f) {
+      case 1: in = cj == ny + 1; if (in) cj = ny; break;
+      case 2: in = ck == 0; if (in) ck = 1; break;
+      case 3: in = cj == 0; if (in) cj = 1; break;
+      case 4: in = ck == nz + 1; if (in) ck = nz; break;
+      case 5: in = ci == 0; if (in) ci = 1; break;
+      default: in = ci == nx + 1; if (in) ci = nx; break;
+    }
+    if (in) chain[nc++] = f;
+  }
+  auto wrap = [](int c, int n) { return c == 0 ? n : (c == n + 1 ? 1 : c); };
+  double x = a[((size_t)wrap(ck, nz) * sy + wrap(cj, ny)) * sx + wrap(ci, nx)];
+  for (int q = nc - 1; q >= 0; --q) {
+    const int f = chain[q];
+    if (bc.type[f] == 2) x = bc.v[f];
+    else if (f == 1 || f == 4 || f == 6) x = x + bc.vd[f];
+    else x = x - bc.vd[f];
+  }
+  a[((size_t)k * sy + j) * sx + i] = x;
 }
 
 // ---------------------------------------------------------------------------
@@ -2761,12 +2796,10 @@ void launch_ghost_wrap(double* a, int nx, int ny, int nz, int dir, hipStream_t s
                  : dir == 1 ? (size_t)(nx + 2) * (nz + 2) : (size_t)(nx + 2) * (ny + 2);
   LAUNCH(k_ghost_wrap, nb(n), kBlock, a, nx, ny, nz, dir);
 }
-void launch_ghost_face(double* a, int nx, int ny, int nz, int face, int type, double v,
-                       double dedge, hipStream_t s) {
-  const size_t n = (face == 1 || face == 3) ? (size_t)(nx + 2) * (nz + 2)
-                 : (face == 2 || face == 4) ? (size_t)(nx + 2) * (ny + 2)
-                                            : (size_t)(ny + 2) * (nz + 2);
-  LAUNCH(k_ghost_face, nb(n), kBlock, a, nx, ny, nz, face, type, v, dedge);
+void launch_ghost_fill(double* a, int nx, int ny, int nz, const GhostBC& bc, hipStream_t s) {
+  const size_t n = 2 * ((size_t)(ny + 2) * (nz + 2) + (size_t)(nx + 2) * (nz + 2) +
+                        (size_t)(nx + 2) * (ny + 2));
+  LAUNCH(k_ghost_fill, nb(n), kBlock, a, nx, ny, nz, bc);
 }
 void launch_psource(const double* U, const double* V, const double* W, double* SRC, const double* rho,
                     const double* rdxc, const double* rdyc, const double* rdzc, int nx, int ny, int nz,

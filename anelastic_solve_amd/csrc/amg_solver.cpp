@@ -1079,15 +1079,16 @@ int Context::finish(const std::vector<Instance*>& S) {
   // ghost layers as mg_bdry(ngrid, phif, .true., .true.) (mgMK/mg_solver.F90:139)
   for (Instance* I : S) {
     const HostLevel& F = I->H.lv[0];
-    double* a = I->d_phi;
-    for (int d = 0; d < 3; ++d) launch_ghost_wrap(a, F.nx, F.ny, F.nz, d, stream);
-    const int* t = cfg.tbc;
-    const double* v = cfg.vbc;
     const double edge[7] = {0.0, F.ay.dxc[F.ny], F.az.dxc[1], F.ay.dxc[1], F.az.dxc[F.nz],
                             F.ax.dxc[1], F.ax.dxc[F.nx]};
-    for (int face = 1; face <= 6; ++face)
-      if (t[face - 1] != AMG_BC_PERIODIC)
-        launch_ghost_face(a, F.nx, F.ny, F.nz, face, t[face - 1], v[face - 1], edge[face], stream);
+    GhostBC bc{};
+    for (int face = 1; face <= 6; ++face) {
+      const int t = cfg.tbc[face - 1];
+      bc.type[face] = t == AMG_BC_NEUMANN ? 1 : t == AMG_BC_DIRICHLET ? 2 : 0;
+      bc.v[face] = cfg.vbc[face - 1];
+      bc.vd[face] = cfg.vbc[face - 1] * edge[face];
+    }
+    launch_ghost_fill(I->d_phi, F.nx, F.ny, F.nz, bc, stream);
   }
   return AMG_OK;
 }

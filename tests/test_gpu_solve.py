@@ -100,6 +100,31 @@ def test_host_path_ghosts_and_warm_start(cuda, precision):
     assert it2 == 0 and s.stats.converged == 1
 
 
+@pytest.mark.parametrize("tbc,vbc", [
+    ((1, 2, 2, 1, 0, 0), (0.3, 0.7, -0.4, 0.25, -1.0, -1.0)),
+    ((2, 1, 1, 2, 1, 2), (0.5, -0.2, 0.1, 1.5, 0.05, -0.3)),
+    ((0, 1, 0, 1, 2, 1), (-1.0, 0.4, -1.0, -0.6, 2.0, 0.15)),
+])
+def test_ghost_layers_every_face_rule(cuda, tbc, vbc):
+    """The returned ghost layers (mgMK/mg_bdry.F90:71-190, all filled in one
+    pass): bitwise the oracle's serial sequence (wraps, then the faces in
+    order) on the returned interior, with Neumann faces of nonzero value,
+    Dirichlet faces and periodic pairs on a stretched grid, from stale
+    ghosts in the caller's array."""
+    from anelastic_solve_amd import PressureSolver, box_grid, default_config
+    from oracle import ops
+    g = box_grid(24, 16, 20, beta=1.5)
+    s = PressureSolver(g, default_config(smoother="LJ", precision="fp64", tbc=tbc, vbc=vbc, maxcy=3))
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(g.shape)
+    phi = rng.standard_normal(g.shape)
+    s.mg_solver(phi, b)
+    levs, _ = oracle_levels(g, tbc, s.levels)
+    ref = ops.fill_ghosts(phi.copy(), list(tbc), list(vbc), levs[0]["ax"], levs[0]["ay"], levs[0]["az"])
+    assert np.array_equal(phi, ref)
+    s.close()
+
+
 def test_dirichlet_quadratic_closed_form(cuda):
     """``dirichletBCs`` of mgMK/test/test.F90:150-189: src = 2 with Dirichlet values
     1 and 2 at the ghost centres of x (periodic y, z) has the exact solution
