@@ -371,14 +371,18 @@ def test_zline_lean_form_full_columns(cuda, precision):
 
 
 @pytest.mark.parametrize("n,tbc", [((128, 128, 128), (0, 1, 0, 1, 0, 0)),
-                                   ((48, 40, 40), (0, 0, 0, 0, 0, 0))])
+                                   ((48, 40, 40), (0, 0, 0, 0, 0, 0)),
+                                   ((32, 32, 256), (0, 1, 0, 1, 0, 0)),
+                                   ((48, 32, 80), (0, 0, 0, 0, 0, 0))])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_zline_half_wave_groups(cuda, precision, n, tbc):
     """A launch of at most half as many column groups as the device has CUs
     runs 32 columns per workgroup, two segments per wave: the 128^3 level of
-    the benchmarked hierarchy (8 full segments per column) and a ragged level
+    the benchmarked hierarchy (8 full segments per column), a ragged level
     (40-cell columns: 3 segments, rounded up to 4; 960 columns per colour, so
-    the last group is partial), periodic z included.  Against the oracle's
+    the last group is partial), the longest columns (256 cells, 16 segments,
+    8 waves per workgroup) and an odd count of full segments (80 cells: 5,
+    rounded up to 6), periodic z included.  Against the oracle's
     Thomas solve per column, and bitwise against the forced forms, which
     keep 64 columns per workgroup."""
     from oracle import vcycle as VC
