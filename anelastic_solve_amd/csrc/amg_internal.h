@@ -43,6 +43,8 @@ struct DevLevel {
                              // exactly while e*hx < 2^32 (the coarse tail's small levels)
   unsigned gmhx, gmny;       // division of any 32-bit index by hx, ny: multiplier and
   int gshx, gsny;            // shifts (s1 | s2 << 8) of gm_magic
+  unsigned gmzp;             // the same for nz rounded up to a multiple of 16 (the direct
+  int gszp;                  // solve's padded z extent)
   int bcx0, bcx1, bcy0, bcy1, bcz0, bcz1;   // boundary codes (prolongation ghosts)
   double vsum;               // finest: sum of cell volumes
   double vrsum;              // finest: sum of V*rho
@@ -150,6 +152,8 @@ struct DevXfer {
                      // post-smoothing sweep
   int rskip;         // kXferSkip*: set for a marching restriction (restrict222_ok)
                      // whose fine level has a pre-smoothing sweep
+  unsigned pmx, pmy; // prolongation: division of a dual index by the duals per
+  int psx, psy;      // x row and per y column (gm_magic multipliers and shifts)
 };
 
 constexpr int kBlock = 256;

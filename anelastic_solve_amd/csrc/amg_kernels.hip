@@ -393,7 +393,7 @@ __device__ __forceinline__ void zline_group(const DevLevel& L, int parity, int z
   const int ncol = hx * ny;
   const bool live = act && col0 < ncol;
   const int col = live ? col0 : ncol - 1;
-  const int m = col % hx, j = col / hx;
+  const int j = gm_div(col, L.gmhx, L.gshx), m = col - j * hx;
   const int c = parity, oth = parity ^ 1;
   const int i = 2 * m + ((c + j) & 1), r = i & 1;
   int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
@@ -1090,11 +1090,13 @@ __device__ __forceinline__ constexpr bool prolong_keeps(int h, int q, int a) {
 
 template <class T, int FX, int FY, int FZ, int SK = kXferSkipNone, class UP, class CUP>
 __device__ __forceinline__ void prolong_dual(const DevLevel& F, UP u, const DevLevel& C, CUP uc,
-                                             int e) {
+                                             const DevXfer& x, int e) {
   static_assert(SK == kXferSkipNone || (FX == 2 && FY == 2 && FZ == 2), "skip: 2:2:2 duals only");
   constexpr int NX = FX == 2 ? 2 : 1, NY = FY == 2 ? 2 : 1, NZ = FZ == 2 ? 2 : 1;
   const int ndx = prolong_duals_dir(C.nx, FX, F.px), ndy = prolong_duals_dir(C.ny, FY, F.py);
-  const int dx = e % ndx, dyz = e / ndx, dy = dyz % ndy, dz = dyz / ndy;
+  // e = (dz * ndy + dy) * ndx + dx, divided by the transfer's invariant divisors
+  const int dyz = gm_div(e, x.pmx, x.psx), dx = e - dyz * ndx;
+  const int dz = gm_div(dyz, x.pmy, x.psy), dy = dyz - dz * ndy;
   const PDir X = pdir<FX>(dx, C.nx, F.px, F.bcx0, F.bcx1);
   const PDir Y = pdir<FY>(dy, C.ny, F.py, F.bcy0, F.bcy1);
   const PDir Z = pdir<FZ>(dz, C.nz, F.pz, F.bcz0, F.bcz1);
@@ -1169,18 +1171,18 @@ __device__ __forceinline__ void prolong_any(const DevLevel& F, UP u, const DevLe
   switch (code) {
     case 7:
       // the skip rules apply to 2:2:2 transfers (DevXfer::skip)
-      if (skip == kXferSkipLine) prolong_dual<T, 2, 2, 2, kXferSkipLine>(F, u, C, uc, e);
-      else if (skip == kXferSkipLineWrap) prolong_dual<T, 2, 2, 2, kXferSkipLineWrap>(F, u, C, uc, e);
-      else if (skip == kXferSkipPoint) prolong_dual<T, 2, 2, 2, kXferSkipPoint>(F, u, C, uc, e);
-      else prolong_dual<T, 2, 2, 2>(F, u, C, uc, e);
+      if (skip == kXferSkipLine) prolong_dual<T, 2, 2, 2, kXferSkipLine>(F, u, C, uc, x, e);
+      else if (skip == kXferSkipLineWrap) prolong_dual<T, 2, 2, 2, kXferSkipLineWrap>(F, u, C, uc, x, e);
+      else if (skip == kXferSkipPoint) prolong_dual<T, 2, 2, 2, kXferSkipPoint>(F, u, C, uc, x, e);
+      else prolong_dual<T, 2, 2, 2>(F, u, C, uc, x, e);
       break;
-    case 6: prolong_dual<T, 2, 2, 1>(F, u, C, uc, e); break;
-    case 5: prolong_dual<T, 2, 1, 2>(F, u, C, uc, e); break;
-    case 4: prolong_dual<T, 2, 1, 1>(F, u, C, uc, e); break;
-    case 3: prolong_dual<T, 1, 2, 2>(F, u, C, uc, e); break;
-    case 2: prolong_dual<T, 1, 2, 1>(F, u, C, uc, e); break;
-    case 1: prolong_dual<T, 1, 1, 2>(F, u, C, uc, e); break;
-    default: prolong_dual<T, 1, 1, 1>(F, u, C, uc, e); break;
+    case 6: prolong_dual<T, 2, 2, 1>(F, u, C, uc, x, e); break;
+    case 5: prolong_dual<T, 2, 1, 2>(F, u, C, uc, x, e); break;
+    case 4: prolong_dual<T, 2, 1, 1>(F, u, C, uc, x, e); break;
+    case 3: prolong_dual<T, 1, 2, 2>(F, u, C, uc, x, e); break;
+    case 2: prolong_dual<T, 1, 2, 1>(F, u, C, uc, x, e); break;
+    case 1: prolong_dual<T, 1, 1, 2>(F, u, C, uc, x, e); break;
+    default: prolong_dual<T, 1, 1, 1>(F, u, C, uc, x, e); break;
   }
 }
 
@@ -1358,7 +1360,7 @@ __global__ __launch_bounds__(NZ == 0 ? 64 * kTailWavesPoint : 64 * kTailWaves) v
       const DevLevel& C = Ls[d + 1];
       const int nd = prolong_duals(L, C, x);
       for (int e = tid; e < nd; e += nthr) {
-        if (ALL222) prolong_dual<T, 2, 2, 2>(L, U(d), C, U(d + 1), e);
+        if (ALL222) prolong_dual<T, 2, 2, 2>(L, U(d), C, U(d + 1), x, e);
         else prolong_any<T>(L, U(d), C, U(d + 1), x, e);
       }
     } else {
@@ -1627,7 +1629,7 @@ __global__ __launch_bounds__(kBlock) void k_fine(DevLevel* __restrict__ Lt, Laun
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = 0.0;
   if (col < plane) {
-    const int m = col % hx, row = col / hx, c = row & 1, j = row >> 1;
+    const int row = gm_div(col, L.gmhx, L.gshx), m = col - row * hx, c = row & 1, j = row >> 1;
     const int i = 2 * m + ((c + j) & 1), r = i & 1, oth = c ^ 1;
     int ml = m - 1 + r; if (ml < 0) ml = hx - 1;
     int mr = m + r;     if (mr >= hx) mr = 0;
@@ -1996,7 +1998,7 @@ __global__ __launch_bounds__(kBlock) void k_wsum(const DevLevel* __restrict__ Lt
   const auto x = gfield<const double>((const void*)(which == 0 ? (L.pcur ? L.u2 : L.u) : L.f));
   double v[1] = {0.0};
   if (col < plane) {
-    const int m = col % hx, row = col / hx, j = row >> 1;
+    const int row = gm_div(col, L.gmhx, L.gshx), m = col - row * hx, j = row >> 1;
     const int i = 2 * m + (((row & 1) + j) & 1);
     const auto vz = vol + nx + ny;
     double s = 0.0;
@@ -2420,13 +2422,21 @@ __global__ __launch_bounds__(kBlock) void k_direct_xy(const DevLevel* __restrict
   auto lmx = [&](int e) { return m[ox + (size_t)(e / kDirectCols) * nx + c0 + e % kDirectCols]; };
   auto smx = [&](int e, CT v) { Mx[(e / kDirectCols) * sc + e % kDirectCols] = v; };
   auto lmy = [&](int e) { return m[oy + e]; };
-  auto smy = [&](int e, CT v) { My[(e / ny) * sy + e % ny] = v; };
-  auto sxp = [&](int e, CT v) { X[(e / nx) * sx + e % nx] = v; };
+  // row and column of a staged element by the level's invariant divisors
+  // (e / nx = (e / 2) / hx)
+  auto smy = [&](int e, CT v) {
+    const int r = gm_div(e, L.gmny, L.gsny);
+    My[r * sy + e - r * ny] = v;
+  };
+  auto sxp = [&](int e, CT v) {
+    const int r = gm_div(e >> 1, L.gmhx, L.gshx);
+    X[r * sx + e - r * nx] = v;
+  };
   const auto scr = gp(reinterpret_cast<CT*>(L.dscr));
   if constexpr (FWD) {
     const auto f = gfield<const T>((const void*)L.f);
     auto lf = [&](int e) {
-      const int j = e / nx, i = e - j * nx;
+      const int j = gm_div(e >> 1, L.gmhx, L.gshx), i = e - j * nx;
       return (CT)f[cidx(i >> 1, j, k, (i + j) & 1, hx, ny)];
     };
     stage3<16, CT>(nx * ny, lf, sxp, ny * ny, lmy, smy, nx * kDirectCols, lmx, smx);
@@ -2470,10 +2480,11 @@ __global__ __launch_bounds__(kBlock) void k_direct_z(const DevLevel* __restrict_
   const size_t oz = (size_t)2 * nx * nx + (size_t)2 * ny * ny;
   const auto fh = gp(reinterpret_cast<const CT*>(L.dscr));
   const auto inv = gp(reinterpret_cast<const CT*>(L.dinv));
+  auto zrow = [&](int e) { return gm_div(e, L.gmzp, L.gszp); };   // e / nzp
   stage3<16, CT>(
-      nzp * nzp, [&](int e) { return m[oz + e]; }, [&](int e, CT v) { Wi[(e / nzp) * sz + e % nzp] = v; },
+      nzp * nzp, [&](int e) { return m[oz + e]; }, [&](int e, CT v) { Wi[zrow(e) * sz + e - zrow(e) * nzp] = v; },
       nzp * nzp, [&](int e) { return m[oz + (size_t)nzp * nzp + e]; },
-      [&](int e, CT v) { W[(e / nzp) * sz + e % nzp] = v; }, nzp * kDirectCols,
+      [&](int e, CT v) { W[zrow(e) * sz + e - zrow(e) * nzp] = v; }, nzp * kDirectCols,
       [&](int e) {
         const int k = min(e / kDirectCols, nz - 1);
         return fh[(size_t)k * plane + m0 + e % kDirectCols];

@@ -303,6 +303,7 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     d.mny = magic((unsigned)d.ny);
     d.gmhx = gm_magic((unsigned)d.hx, &d.gshx);
     d.gmny = gm_magic((unsigned)d.ny, &d.gsny);
+    d.gmzp = gm_magic((unsigned)((d.nz + 15) & ~15), &d.gszp);
     d.px = I->H.bc[0][0] == AMG_BC_PERIODIC;
     d.py = I->H.bc[1][0] == AMG_BC_PERIODIC;
     d.pz = I->H.bc[2][0] == AMG_BC_PERIODIC;
@@ -405,6 +406,8 @@ int Context::add_instance(const amg_grid& g, Instance** out, Hierarchy* built) {
     x.rskip = kXferSkipNone;
     if (I->pre[l] >= 1 && restrict222_ok(ents[l], ents[l + 1], x) && !(line && ents[l].pz))
       x.rskip = line ? kXferSkipLine : kXferSkipPoint;
+    x.pmx = gm_magic((unsigned)prolong_duals_dir(ents[l + 1].nx, x.fx, ents[l].px), &x.psx);
+    x.pmy = gm_magic((unsigned)prolong_duals_dir(ents[l + 1].ny, x.fy, ents[l].py), &x.psy);
     I->xf.push_back(xbase + (int)newX.size());
     newX.push_back(x);
   }
