@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict222(const DevLevel* __restri
   const int lb = b - ll.prefix[t];
   const int bk = lb / ncb, col = (lb - bk * ncb) * kBlock + threadIdx.x;
   if (col >= cols) return;
-  const int J = col / C.nx, I = col - J * C.nx;
+  const int J = gm_div(col >> 1, C.gmhx, C.gshx), I = col - J * C.nx;   // col / nx, nx = 2 hx
   const int kr = restrict222_k(C);
   const int K0 = bk * kr, K1 = min(K0 + kr, C.nz);
   const int sk = use_skip ? x.rskip : kXferSkipNone;
@@ -2103,19 +2103,21 @@ __global__ void k_ghost_wrap(double* __restrict__ a, int nx, int ny, int nz, int
 // product made on the host so the rule is one rounding, x -/+ vd).
 __global__ __launch_bounds__(kBlock) void k_ghost_fill(double* __restrict__ a, int nx, int ny, int nz,
                                                        GhostBC bc) {
+  // (32-bit index arithmetic: the ghost cells of any grid this library
+  // accepts number far below 2^31)
   const int sx = nx + 2, sy = ny + 2, sz = nz + 2;
-  const long long nxf = 2LL * sy * sz, nyf = 2LL * sx * sz, nzf = 2LL * sx * sy;
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nxf = 2 * sy * sz, nyf = 2 * sx * sz, nzf = 2 * sx * sy;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
   int i, j, k;
   if (e < nxf) {
-    const long long r = e >> 1;
-    i = (e & 1) ? nx + 1 : 0; j = (int)(r % sy); k = (int)(r / sy);
+    const int r = e >> 1;
+    i = (e & 1) ? nx + 1 : 0; k = r / sy; j = r - k * sy;
   } else if (e < nxf + nyf) {
-    const long long r = (e - nxf) >> 1;
-    j = ((e - nxf) & 1) ? ny + 1 : 0; i = (int)(r % sx); k = (int)(r / sx);
+    const int r = (e - nxf) >> 1;
+    j = ((e - nxf) & 1) ? ny + 1 : 0; k = r / sx; i = r - k * sx;
   } else if (e < nxf + nyf + nzf) {
-    const long long r = (e - nxf - nyf) >> 1;
-    k = ((e - nxf - nyf) & 1) ? nz + 1 : 0; i = (int)(r % sx); j = (int)(r / sx);
+    const int r = (e - nxf - nyf) >> 1;
+    k = ((e - nxf - nyf) & 1) ? nz + 1 : 0; j = r / sx; i = r - j * sx;
   } else {
     return;
   }
