@@ -100,11 +100,27 @@ def test_host_path_ghosts_and_warm_start(cuda, precision):
     assert it2 == 0 and s.stats.converged == 1
 
 
+def _face_rule_cases():
+    # a seeded sample (about 40 %) of the (x, y, z) choices of a periodic
+    # pair or a Neumann/Dirichlet type per face (faces: 1 y+, 2 z-, 3 y-,
+    # 4 z+, 5 x-, 6 x+), with seeded face values
+    import itertools
+    rng = np.random.default_rng(12)
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (2, 1)]
+    out = []
+    for (x0, x1), (y0, y1), (z0, z1) in itertools.product(pairs, repeat=3):
+        if (x0, y0, z0) == (0, 0, 0) or rng.random() < 0.6:
+            continue
+        tbc = (y1, z0, y0, z1, x0, x1)
+        out.append((tbc, tuple(float(v) for v in rng.uniform(-1.0, 1.0, 6))))
+    return out
+
+
 @pytest.mark.parametrize("tbc,vbc", [
     ((1, 2, 2, 1, 0, 0), (0.3, 0.7, -0.4, 0.25, -1.0, -1.0)),
     ((2, 1, 1, 2, 1, 2), (0.5, -0.2, 0.1, 1.5, 0.05, -0.3)),
     ((0, 1, 0, 1, 2, 1), (-1.0, 0.4, -1.0, -0.6, 2.0, 0.15)),
-])
+] + _face_rule_cases())
 def test_ghost_layers_every_face_rule(cuda, tbc, vbc):
     """The returned ghost layers (mgMK/mg_bdry.F90:71-190, all filled in one
     pass): bitwise the oracle's serial sequence (wraps, then the faces in
